@@ -1,0 +1,100 @@
+/*
+ * intc.h -- C-ABI of libintc.so, the MI355X (gfx950) contracted-ERI library.
+ *
+ * All functions return 0 on success and a negative status on error
+ * (-1 invalid argument, -2 class not built into the library, -3 HIP error);
+ * intc_last_error() gives the message.  The reference's generated functions
+ * return void and do no checking (src/intception/generator/callables/
+ * main_source_algorithms/contracted_general.py:31-73); these return a status
+ * instead and otherwise keep the reference's argument layout and meaning.
+ *
+ * Reference names: "<full>" below stands for the prefix the reference
+ * generator gives the ERI class of the input script, e.g.
+ * intception_integral_eri_abcd_m_cont_sph (integral name from
+ * src/intception/dsl_extensions.py:612-627, suffixes from
+ * src/intception/generator/generator.py:432-457).
+ */
+#ifndef INTC_H
+#define INTC_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------------- */
+/* Library state                                                          */
+/* ---------------------------------------------------------------------- */
+const char *intc_last_error(void);
+int intc_version(void);
+/* Uploads the Boys tables to the current device (also done lazily). */
+int intc_init(void);
+/* 0: Chebyshev interpolation + downward recursion (default).
+ * 1: the reference's three-branch scheme
+ *    (src/intception/generator/callables/generator_boys_function.py:98-205),
+ *    for bit-level parity studies. */
+int intc_set_boys_mode(int mode);
+/* 1 if the canonical form of (la lb|lc ld) has a kernel in this build. */
+int intc_class_supported(int la, int lb, int lc, int ld, int spherical);
+
+/* ---------------------------------------------------------------------- */
+/* Per-quartet twins of the generated functions (host pointers)           */
+/* ---------------------------------------------------------------------- */
+/* Replaces <full>_work_array_size
+ * (src/intception/generator/callables/main_source_algorithms/contracted_vrr_hrr_with_auxiliary.py:248-351).
+ * Returns the same bound as the reference's formula; the device path itself
+ * does not use the caller's work array. */
+int intc_eri_cont_sph_work_array_size(int la, int lb, int lc, int ld, int na_prim, int nb_prim, int nc_prim,
+                                      int nd_prim, int na_cont, int nb_cont, int nc_cont, int nd_cont);
+int intc_eri_cont_cart_work_array_size(int la, int lb, int lc, int ld, int na_prim, int nb_prim, int nc_prim,
+                                       int nd_prim, int na_cont, int nb_cont, int nc_cont, int nd_cont);
+
+#define INTC_SHELL_ARGS(X)                                                                            \
+    const double *x##X, const double c##X[3], const int l##X, const int n##X, const int iskip##X,     \
+        const int n##X##_prim, const int n##X##_cont, const int iskip##X##_cont,                      \
+        const double *contract_array_##X
+
+/* Replaces <full>_main (spherical variant): argument order per index is
+ * exponents, centre, l, n, iskip, n_prim, n_cont, iskip_cont,
+ * contract_array (n_prim x n_cont, column-major), sph_trans_array
+ * (ncc(l) x (2l+1), column-major) -- contracted_general.py:31-73 and
+ * src/intception/generator/integral_wrapper.py:1892-1926.  Output element
+ * out[sum_X(iX*iskipX + icontX*iskipX_cont)] as in contracted_general.py:1808-1875. */
+int intc_eri_cont_sph_main(INTC_SHELL_ARGS(a), const double *sph_trans_array_a, INTC_SHELL_ARGS(b),
+                           const double *sph_trans_array_b, INTC_SHELL_ARGS(c), const double *sph_trans_array_c,
+                           INTC_SHELL_ARGS(d), const double *sph_trans_array_d, double *work_array,
+                           double *output_array);
+/* Replaces <full>_main (Cartesian variant, no sph_trans_array arguments). */
+int intc_eri_cont_cart_main(INTC_SHELL_ARGS(a), INTC_SHELL_ARGS(b), INTC_SHELL_ARGS(c), INTC_SHELL_ARGS(d),
+                            double *work_array, double *output_array);
+
+/* ---------------------------------------------------------------------- */
+/* Batched twin (no counterpart in the reference, which evaluates one     */
+/* quartet per call): the same fields for many quartets, as an SoA shell  */
+/* table.  Shell s has angular momentum shell_l[s], shell_nprim[s]        */
+/* primitives at exps/coefs[shell_off[s] ...] (n_cont = 1) and centre     */
+/* shell_cen[3s..3s+2]; quartet q is shells quartets[4q..4q+3] (a,b,c,d)  */
+/* and writes a row-major [a][b][c][d] block at out[out_off[q]].          */
+/* ---------------------------------------------------------------------- */
+typedef struct intc_plan intc_plan;
+
+/* Host arrays; bins quartets by canonical class and primitive count and
+ * uploads the index data.  Device memory is owned by the plan. */
+int intc_plan_create(intc_plan **plan, int spherical, int n_shell, const int *shell_l, const int *shell_nprim,
+                     const int *shell_off, const double *shell_cen, long long n_quartet, const int *quartets,
+                     const long long *out_off);
+int intc_plan_num_bins(const intc_plan *plan);
+int intc_plan_bin_info(const intc_plan *plan, int bin, int *cls4, int *nq, int *team_groups, int *grid, int *G);
+int intc_plan_set_bin_tuning(intc_plan *plan, int bin, int team_groups, int grid);
+/* Device pointers.  sph: S_l for l = 0..7 concatenated, each ncc(l) x (2l+1)
+ * column-major (NULL for Cartesian plans).  stream: hipStream_t or NULL. */
+int intc_plan_build_pairs(intc_plan *plan, const double *exps, const double *coefs, const double *shell_cen,
+                          void *stream);
+int intc_plan_run_bin(intc_plan *plan, int bin, const double *sph, double *out, void *stream);
+int intc_plan_execute(intc_plan *plan, const double *exps, const double *coefs, const double *shell_cen,
+                      const double *sph, double *out, void *stream);
+int intc_plan_destroy(intc_plan *plan);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,88 @@
+"""Build libintc.so (gfx950) from the hand-written runtime and the generated
+per-class kernels.
+
+    python -m intception_amd.build [--lmax 2] [--jobs 8]
+
+Steps: Boys tables (mpmath) -> class kernels (codegen) -> hipcc -c per file
+(parallel) -> link.  Objects are reused when their source is unchanged.
+"""
+import argparse
+import hashlib
+import os
+import subprocess
+import sys
+from concurrent.futures import ThreadPoolExecutor
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+GEN = os.path.join(CSRC, "gen")
+OBJ = os.path.join(HERE, "_build")
+LIB = os.path.join(HERE, "libintc.so")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = "gfx950"
+FLAGS = ["-O3", "-fPIC", "-std=c++17", "--offload-arch=" + ARCH,
+         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+         "-Wno-unused-command-line-argument"]
+
+# classes compiled into the library: every canonical class with shells up to
+# l = 2 (Cartesian and spherical), plus the three-centre and high-l sets
+EXTRA_CLASSES = []
+
+
+def _digest(paths, extra=""):
+    h = hashlib.sha1(extra.encode())
+    for p in paths:
+        with open(p, "rb") as fh:
+            h.update(fh.read())
+    return h.hexdigest()
+
+
+def _compile(src, deps):
+    os.makedirs(OBJ, exist_ok=True)
+    base = os.path.splitext(os.path.basename(src))[0]
+    obj = os.path.join(OBJ, base + ".o")
+    stamp = obj + ".sha"
+    dig = _digest([src] + deps, " ".join(FLAGS))
+    if os.path.exists(obj) and os.path.exists(stamp) and open(stamp).read() == dig:
+        return obj, False
+    cmd = [HIPCC] + FLAGS + ["-I", CSRC, "-c", src, "-o", obj]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("hipcc failed for %s:\n%s" % (src, r.stderr[-6000:]))
+    with open(stamp, "w") as fh:
+        fh.write(dig)
+    return obj, True
+
+
+def build(lmax=2, jobs=None, verbose=True):
+    from . import boys_tables, codegen
+    inc = os.path.join(CSRC, "boys_tables.inc")
+    if not os.path.exists(inc):
+        boys_tables.write_inc(inc)
+    classes = codegen.class_set(lmax, EXTRA_CLASSES)
+    files, metas = codegen.generate(GEN, classes)
+    common = [os.path.join(CSRC, "intc_common.h")]
+    srcs = [(f, common) for f in files]
+    srcs.append((os.path.join(CSRC, "intc_runtime.hip"),
+                 common + [inc, os.path.join(GEN, "class_table.inc")]))
+    jobs = jobs or min(16, os.cpu_count() or 4)
+    with ThreadPoolExecutor(jobs) as ex:
+        res = list(ex.map(lambda a: _compile(*a), srcs))
+    objs = [r[0] for r in res]
+    rebuilt = sum(1 for r in res if r[1])
+    cmd = [HIPCC, "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", LIB + ".tmp"] + objs
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("link failed:\n" + r.stderr[-4000:])
+    os.replace(LIB + ".tmp", LIB)
+    if verbose:
+        print("libintc.so: %d classes x 2 outputs, %d objects rebuilt" % (len(classes), rebuilt))
+    return LIB
+
+
+if __name__ == "__main__":
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--lmax", type=int, default=2)
+    ap.add_argument("--jobs", type=int, default=None)
+    a = ap.parse_args()
+    build(a.lmax, a.jobs)

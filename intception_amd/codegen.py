@@ -1,0 +1,431 @@
+"""HIP backend: emits one gfx950 kernel per canonical ERI class.
+
+The recurrences are those of the reference's ERI input script -- Obara-Saika
+VRR (bra index a, ket index c, one auxiliary index m) and the Head-Gordon-Pople
+HRR (src/intception/dsl_extensions.py:822-950) -- with the reference's direction
+rule, "first non-zero component of the target"
+(src/intception/generator/generator.py:547-558).  The schedule is MI355X's
+own rather than the reference's 2-layer work-array scheme
+(src/intception/generator/callables/main_source_algorithms/contracted_general.py:720-1080):
+
+* lanes over bra components.  A *group* of G = ncc0(la+lb) lanes works on one
+  primitive quartet; lane e owns bra component e (all |e| <= la+lb).  Lane e
+  builds (e 0|0 0)^(m) along the chain 0 -> ... -> e of the direction rule,
+  with per-lane direction selects (one straight-line program for all lanes).
+* ket in registers.  Every lane then runs the same unrolled OS recursion on the
+  ket index c, (e 0|f 0)^(m) for |f| <= lc+ld; the only cross-lane term,
+  e_i/(2(zeta+eta)) (e-1_i 0|f 0)^(m+1), is a ds_bpermute from the lane that
+  owns e-1_i.
+* contraction in registers.  Coefficients are folded into the pair prefactor,
+  so contraction is a plain sum over primitive quartets; a *team* of T groups
+  splits the primitive quartets of one contracted quartet and is reduced with
+  cross-lane reads (tree over groups).
+* HRR and transform.  Group 0 applies the ket HRR (c,d) and the ket
+  Cartesian->spherical step in registers, writes (e|cd) to LDS, and the team's
+  lanes take one cd column each for the bra HRR (a,b) and the bra transform,
+  writing the output rows coalesced.
+
+Canonical classes have la >= lb, lc >= ld and the smaller pair in the bra
+(lanes); the batch planner permutes shells and output strides accordingly.
+"""
+import itertools
+import os
+
+from .basis import ncc, ncc0, nsph, cart_components
+
+
+def comps_upto(l):
+    out = []
+    for k in range(l + 1):
+        out += cart_components(k)
+    return out
+
+
+def cidx(c):
+    """0-based index of component c among all components with |c'| <= |c|."""
+    x, y, z = c
+    l = x + y + z
+    yz = y + z
+    return ncc0(l - 1) + yz * (yz + 1) // 2 + z
+
+
+def first_nonzero(c):
+    for i in range(3):
+        if c[i] > 0:
+            return i
+    return 0
+
+
+def dec(c, i, k=1):
+    c = list(c)
+    c[i] -= k
+    return tuple(c)
+
+
+def canonical(la, lb, lc, ld):
+    """Canonical class and the permutation applied.
+
+    Returns (cls, perm) where cls = (LA, LB, LC, LD) and perm[k] is the original
+    position (0..3 for a,b,c,d) of canonical position k."""
+    bra, ket = [(la, 0), (lb, 1)], [(lc, 2), (ld, 3)]
+    if bra[0][0] < bra[1][0]:
+        bra = [bra[1], bra[0]]
+    if ket[0][0] < ket[1][0]:
+        ket = [ket[1], ket[0]]
+    kb = (bra[0][0] + bra[1][0], bra[0][0])
+    kk = (ket[0][0] + ket[1][0], ket[0][0])
+    if kb > kk:
+        bra, ket = ket, bra
+    cls = (bra[0][0], bra[1][0], ket[0][0], ket[1][0])
+    perm = (bra[0][1], bra[1][1], ket[0][1], ket[1][1])
+    return cls, perm
+
+
+def canonical_classes(lmax):
+    out = set()
+    for c in itertools.product(range(lmax + 1), repeat=4):
+        out.add(canonical(*c)[0])
+    return sorted(out)
+
+
+class Emitter:
+    def __init__(self):
+        self.lines = []
+        self.ind = 1
+
+    def __call__(self, s=""):
+        self.lines.append("    " * self.ind + s)
+
+    def text(self):
+        return "\n".join(self.lines)
+
+
+def kernel_name(cls, spherical, prec="d"):
+    return "intc_eri_%s%d%d%d%d_%s" % (prec, *cls, "s" if spherical else "c")
+
+
+def _chain_lines(LX, M):
+    """Bra chain: c{s}_{m} for s = 0..LX, m = 0..M-s; c0 given."""
+    L = []
+    for s in range(1, LX + 1):
+        L.append("const int d%d = (cdir >> %d) & 3;" % (s, 2 * (s - 1)))
+        L.append("const double pa%d = d%d == 0 ? PAx : (d%d == 1 ? PAy : (d%d == 2 ? PAz : 1.0));" % ((s,) * 4))
+        L.append("const double wp%d = d%d == 0 ? WPx : (d%d == 1 ? WPy : (d%d == 2 ? WPz : 0.0));" % ((s,) * 4))
+        if s >= 2:
+            L.append("const double kk%d = (double)((ckk >> %d) & 15) * o2z;" % (s, 4 * (s - 1)))
+        for m in range(M - s + 1):
+            expr = "pa%d * c%d_%d + wp%d * c%d_%d" % (s, s - 1, m, s, s - 1, m + 1)
+            if s >= 2:
+                expr += " + kk%d * (c%d_%d - roz * c%d_%d)" % (s, s - 2, m, s - 2, m + 1)
+            L.append("const double c%d_%d = %s;" % (s, m, expr))
+    return L
+
+
+def _ket_lines(LX, LR, foff, prec="d"):
+    """Ket OS VRR in registers: V{f}_{m} for |f| <= LR, m <= LR-|f|.
+
+    Returns (lines, name(f, m))."""
+    L = []
+    def V(f, m):
+        return "v%d_%d" % (cidx(f), m)
+    for m in range(LR + 1):
+        L.append("const double %s = c%d_%d;" % (V((0, 0, 0), m), LX, m))
+    for n in range(1, LR + 1):
+        for t in cart_components(n):
+            i = first_nonzero(t)
+            f1 = dec(t, i)
+            k1 = f1[i]
+            ax = "xyz"[i]
+            for m in range(LR - n + 1):
+                expr = "QC%s * %s + WQ%s * %s" % (ax, V(f1, m), ax, V(f1, m + 1))
+                if k1 > 0:
+                    f2 = dec(f1, i)
+                    expr += " + %d.0 * o2e * (%s - roe * %s)" % (k1, V(f2, m), V(f2, m + 1))
+                if LX > 0:
+                    expr += " + cpl%s * intc_shfl(%s, s4%s)" % (ax, V(f1, m + 1), ax)
+                L.append("const double %s = %s;" % (V(t, m), expr))
+    return L, V
+
+
+def _hrr_lines(src, lo, nmove, shift_vec, prefix, kind):
+    """Generic HGP HRR on one index pair.
+
+    src(e) gives the expression of (e, 0) for |e| in [lo, lo+nmove]; returns
+    lines and a function H(a, b) giving (a, b) for |a| = lo, |b| = nmove.
+    (a, b+1_i) = (a+1_i, b) + R_i (a, b), i = first non-zero of the target b.
+    """
+    L = []
+    names = {}
+    for e in comps_upto(lo + nmove):
+        if sum(e) >= lo:
+            names[(e, (0, 0, 0))] = src(e)
+    for bl in range(1, nmove + 1):
+        for e in comps_upto(lo + nmove - bl):
+            if sum(e) < lo:
+                continue
+            for b in cart_components(bl):
+                i = first_nonzero(b)
+                b1 = dec(b, i)
+                e1 = list(e); e1[i] += 1; e1 = tuple(e1)
+                nm = "%s%d_%d" % (prefix, cidx(e), cidx(b))
+                L.append("const double %s = %s + %s%s * %s;" % (nm, names[(e1, b1)], shift_vec, "xyz"[i],
+                                                                names[(e, b1)]))
+                names[(e, b)] = nm
+    return L, (lambda a_, b_: names[(a_, b_)])
+
+
+def emit_class_source(cls, spherical):
+    LA, LB, LC, LD = cls
+    LX, LR = LA + LB, LC + LD
+    M = LX + LR
+    G = ncc0(LX)
+    eoff = ncc0(LA - 1)
+    NE = G - eoff
+    nC = nsph(LC) if spherical else ncc(LC)
+    nD = nsph(LD) if spherical else ncc(LD)
+    NCD = nC * nD
+    nA = nsph(LA) if spherical else ncc(LA)
+    nB = nsph(LB) if spherical else ncc(LB)
+    name = kernel_name(cls, spherical)
+    bra_comps = comps_upto(LX)
+    ket_comps = comps_upto(LR)
+
+    lane_tab = []
+    for e in bra_comps:
+        src = [cidx(dec(e, i)) if e[i] > 0 else cidx(e) for i in range(3)]
+        steps = []
+        t = e
+        while sum(t) > 0:
+            i = first_nonzero(t)
+            steps.append((i, t[i] - 1))
+            t = dec(t, i)
+        steps = steps[::-1]
+        dirs, ks = 0, 0
+        for s in range(LX):
+            d, k = steps[s] if s < len(steps) else (3, 0)
+            dirs |= d << (2 * s)
+            ks |= k << (4 * s)
+        lane_tab.append((e[0], e[1], e[2], src[0], src[1], src[2], dirs, ks))
+
+    o = Emitter()
+    o.ind = 0
+    o("// Generated by intception_amd/codegen.py -- do not edit.")
+    o("// Canonical class (%d %d | %d %d), %s output, FP64." % (LA, LB, LC, LD,
+                                                                "spherical" if spherical else "Cartesian"))
+    o('#include "../intc_common.h"')
+    o("")
+    o("__constant__ int %s_lane[%d][8] = {" % (name, G))
+    for r in lane_tab:
+        o("    {%s}," % ", ".join(str(v) for v in r))
+    o("};")
+    o("")
+    o('extern "C" __global__ void __launch_bounds__(256) %s(const IntcEriArgs a)' % name)
+    o("{")
+    o.ind = 1
+    o("constexpr int G = %d, NE = %d, NCD = %d, M = %d;" % (G, NE, NCD, M))
+    o("extern __shared__ double lds[];")
+    o("const int T = a.team_groups, TL = G * T, TPW = 64 / TL;")
+    o("const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;")
+    o("const int tw = lane / TL, tl = lane - tw * TL;")
+    o("const int grp = tl / G, e = tl - grp * G;")
+    o("if (tw >= TPW) return;")
+    o("double *tlds = lds + (size_t)(wave * TPW + tw) * (NE * NCD);")
+    o("const int *lt = %s_lane[e];" % name)
+    o("const double ex_d = (double)lt[0], ey_d = (double)lt[1], ez_d = (double)lt[2];")
+    o("const int lbase = lane - e;")
+    o("const int s4x = (lbase + lt[3]) * 4, s4y = (lbase + lt[4]) * 4, s4z = (lbase + lt[5]) * 4;")
+    o("const int cdir = lt[6], ckk = lt[7];")
+    o("(void)ex_d; (void)ey_d; (void)ez_d; (void)s4x; (void)s4y; (void)s4z; (void)cdir; (void)ckk;")
+    o("const int nteams = gridDim.x * (blockDim.x >> 6) * TPW;")
+    o("for (int q = (blockIdx.x * (blockDim.x >> 6) + wave) * TPW + tw; q < a.nq; q += nteams) {")
+    o.ind = 2
+    o("const int pb = a.q_bra[q], pk = a.q_ket[q];")
+    o("const int ob = a.pair_off[pb], Pb = a.pair_np[pb];")
+    o("const int okk = a.pair_off[pk], Pk = a.pair_np[pk];")
+    o("const double Ax = a.pair_A[3 * pb], Ay = a.pair_A[3 * pb + 1], Az = a.pair_A[3 * pb + 2];")
+    o("const double Cx = a.pair_A[3 * pk], Cy = a.pair_A[3 * pk + 1], Cz = a.pair_A[3 * pk + 2];")
+    o("(void)Ax; (void)Ay; (void)Az; (void)Cx; (void)Cy; (void)Cz;")
+    o("const int P = Pb * Pk;")
+    acc_f = [f for f in ket_comps if sum(f) >= LC]
+    for f in acc_f:
+        o("double acc%d = 0.0;" % cidx(f))
+    o("int ib = grp / Pk, ik = grp - ib * Pk;")
+    o("for (int p = grp; p < P; p += T) {")
+    o.ind = 3
+    o("const double zeta = a.pp_zeta[ob + ib], o2z = a.pp_o2z[ob + ib];")
+    o("const double Px = a.pp_px[ob + ib], Py = a.pp_py[ob + ib], Pz = a.pp_pz[ob + ib];")
+    o("const double kab = a.pp_kappa[ob + ib];")
+    o("const double eta = a.pp_zeta[okk + ik], o2e = a.pp_o2z[okk + ik];")
+    o("const double Qx = a.pp_px[okk + ik], Qy = a.pp_py[okk + ik], Qz = a.pp_pz[okk + ik];")
+    o("const double kcd = a.pp_kappa[okk + ik];")
+    o("(void)o2z; (void)o2e;")
+    o("ik += T; while (ik >= Pk) { ik -= Pk; ++ib; }")
+    o("const double os = 1.0 / (zeta + eta);")
+    o("const double rho = zeta * eta * os;")
+    o("const double PQx = Px - Qx, PQy = Py - Qy, PQz = Pz - Qz;")
+    o("const double Tb = rho * (PQx * PQx + PQy * PQy + PQz * PQz);")
+    o("const double pref = kab * kcd * sqrt(os);")
+    o("double F[M + 1];")
+    o("intc_boys<M>(Tb, F, a);")
+    if LX > 0:
+        o("const double eo = eta * os;")
+        o("const double PAx = Px - Ax, PAy = Py - Ay, PAz = Pz - Az;")
+        o("const double WPx = -eo * PQx, WPy = -eo * PQy, WPz = -eo * PQz;")
+        o("const double roz = eo;")
+    if LR > 0:
+        o("const double zo = zeta * os;")
+        o("const double QCx = Qx - Cx, QCy = Qy - Cy, QCz = Qz - Cz;")
+        o("const double WQx = zo * PQx, WQy = zo * PQy, WQz = zo * PQz;")
+        o("const double roe = zo;")
+        if LX > 0:
+            o("const double o2s = 0.5 * os;")
+            o("const double cplx = ex_d * o2s, cply = ey_d * o2s, cplz = ez_d * o2s;")
+    for m in range(M + 1):
+        o("const double c0_%d = pref * F[%d];" % (m, m))
+    for ln in _chain_lines(LX, M):
+        o(ln)
+    lines, V = _ket_lines(LX, LR, ncc0(LC - 1))
+    for ln in lines:
+        o(ln)
+    for f in acc_f:
+        o("acc%d += %s;" % (cidx(f), V(f, 0)))
+    o.ind = 2
+    o("}")
+    # team reduction over groups (tree towards group 0)
+    o("for (int s = 1; s < T; s <<= 1) {")
+    o.ind = 3
+    o("const int src4 = (lane + s * G < 64 ? lane + s * G : lane) * 4;")
+    o("const bool take = ((grp & (2 * s - 1)) == 0) && (grp + s < T);")
+    for f in acc_f:
+        o("{ const double r = intc_shfl(acc%d, src4); if (take) acc%d += r; }" % (cidx(f), cidx(f)))
+    o.ind = 2
+    o("}")
+    o("const long long qoff = a.q_off[q];")
+    o("const unsigned long long qs = a.q_str[q];")
+    o("const int sA = (int)(qs & 0xffff), sB = (int)((qs >> 16) & 0xffff);")
+    o("const int sC = (int)((qs >> 32) & 0xffff), sD = (int)((qs >> 48) & 0xffff);")
+    o("(void)sA; (void)sB;")
+    # ket HRR + ket sph by group 0 lanes with |e| >= LA
+    o("if (grp == 0 && e >= %d) {" % eoff)
+    o.ind = 3
+    if LD > 0:
+        o("const double CDx = a.pair_AB[3 * pk], CDy = a.pair_AB[3 * pk + 1], CDz = a.pair_AB[3 * pk + 2];")
+    hl, H = _hrr_lines(lambda f: "acc%d" % cidx(f), LC, LD, "CD", "hk", "ket")
+    for ln in hl:
+        o(ln)
+    cc, dc = cart_components(LC), cart_components(LD)
+    if spherical:
+        sC, sD = "a.sph + %d" % intc_sph_off(LC), "a.sph + %d" % intc_sph_off(LD)
+        o("const double *SC = %s, *SD = %s;" % (sC, sD))
+        # d first: K1(c, sd) = sum_d H(c,d) SD[d + sd*ncc(LD)]
+        for ci, c in enumerate(cc):
+            for sd in range(nsph(LD)):
+                terms = " + ".join("%s * SD[%d]" % (H(c, d), di + sd * ncc(LD)) for di, d in enumerate(dc))
+                o("const double k1_%d_%d = %s;" % (ci, sd, terms))
+        for sc in range(nsph(LC)):
+            for sd in range(nsph(LD)):
+                terms = " + ".join("SC[%d] * k1_%d_%d" % (ci + sc * ncc(LC), ci, sd) for ci in range(len(cc)))
+                o("tlds[(e - %d) * NCD + %d] = %s;" % (eoff, sc * nD + sd, terms))
+    else:
+        for ci, c in enumerate(cc):
+            for di, d in enumerate(dc):
+                o("tlds[(e - %d) * NCD + %d] = %s;" % (eoff, ci * nD + di, H(c, d)))
+    o.ind = 2
+    o("}")
+    o('__builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");')
+    o("__builtin_amdgcn_wave_barrier();")
+    o('__builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");')
+    # column phase: bra HRR + bra sph
+    o("for (int j = tl; j < NCD; j += TL) {")
+    o.ind = 3
+    o("const int jc = j / %d, jd = j - jc * %d;" % (nD, nD))
+    o("double *op = a.out + qoff + (long long)jc * sC + (long long)jd * sD;")
+    bra_keep = [ec for ec in bra_comps if sum(ec) >= LA]
+    for ec in bra_keep:
+        o("const double x%d = tlds[%d * NCD + j];" % (cidx(ec), cidx(ec) - eoff))
+    if LB > 0:
+        o("const double ABx = a.pair_AB[3 * pb], ABy = a.pair_AB[3 * pb + 1], ABz = a.pair_AB[3 * pb + 2];")
+    hl, HB = _hrr_lines(lambda ec: "x%d" % cidx(ec), LA, LB, "AB", "hb", "bra")
+    for ln in hl:
+        o(ln)
+    ac, bc = cart_components(LA), cart_components(LB)
+    if spherical:
+        o("const double *SA = a.sph + %d, *SB = a.sph + %d;" % (intc_sph_off(LA), intc_sph_off(LB)))
+        for ai, aa in enumerate(ac):
+            for sb in range(nsph(LB)):
+                terms = " + ".join("%s * SB[%d]" % (HB(aa, bb), bi + sb * ncc(LB)) for bi, bb in enumerate(bc))
+                o("const double m1_%d_%d = %s;" % (ai, sb, terms))
+        for sa in range(nsph(LA)):
+            for sb in range(nsph(LB)):
+                terms = " + ".join("SA[%d] * m1_%d_%d" % (ai + sa * ncc(LA), ai, sb) for ai in range(len(ac)))
+                o("op[%d * sA + %d * sB] = %s;" % (sa, sb, terms))
+    else:
+        for ai, aa in enumerate(ac):
+            for bi, bb in enumerate(bc):
+                o("op[%d * sA + %d * sB] = %s;" % (ai, bi, HB(aa, bb)))
+    o.ind = 2
+    o("}")
+    o('__builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");')
+    o("__builtin_amdgcn_wave_barrier();")
+    o.ind = 1
+    o("}")
+    o.ind = 0
+    o("}")
+    o("")
+    o('extern "C" hipError_t %s_launch(const IntcEriArgs *a, int grid, int block, size_t lds, hipStream_t st)'
+      % name)
+    o("{")
+    o("    hipLaunchKernelGGL(%s, dim3(grid), dim3(block), lds, st, *a);" % name)
+    o("    return hipGetLastError();")
+    o("}")
+    meta = dict(cls=cls, spherical=spherical, G=G, NE=NE, NCD=NCD, M=M, name=name)
+    return o.text() + "\n", meta
+
+
+def intc_sph_off(l):
+    return sum(ncc(k) * (2 * k + 1) for k in range(l))
+
+
+def class_set(lmax_pair_side=2, extra=()):
+    """Canonical classes for shells up to lmax, plus extra (la,lb,lc,ld)."""
+    out = set(canonical_classes(lmax_pair_side))
+    for c in extra:
+        out.add(canonical(*c)[0])
+    return sorted(out)
+
+
+def generate(outdir, classes, sph_modes=(False, True)):
+    os.makedirs(outdir, exist_ok=True)
+    metas = []
+    written = []
+    for cls in classes:
+        for sph in sph_modes:
+            src, meta = emit_class_source(cls, sph)
+            fn = os.path.join(outdir, meta["name"] + ".hip")
+            old = open(fn).read() if os.path.exists(fn) else None
+            if old != src:
+                with open(fn, "w") as fh:
+                    fh.write(src)
+            metas.append(meta)
+            written.append(fn)
+    # dispatch table
+    lines = ["// Generated by intception_amd/codegen.py -- do not edit.", '#include "../intc_common.h"',
+             "typedef hipError_t (*intc_launch_fn)(const IntcEriArgs *, int, int, size_t, hipStream_t);"]
+    for m in metas:
+        lines.append('extern "C" hipError_t %s_launch(const IntcEriArgs *, int, int, size_t, hipStream_t);'
+                     % m["name"])
+    lines.append("struct IntcClassEntry { int la, lb, lc, ld, sph, G, NE, NCD, M; intc_launch_fn fn; };")
+    lines.append("static const IntcClassEntry intc_class_table[] = {")
+    for m in metas:
+        la, lb, lc, ld = m["cls"]
+        lines.append("    {%d, %d, %d, %d, %d, %d, %d, %d, %d, %s_launch}," % (
+            la, lb, lc, ld, int(m["spherical"]), m["G"], m["NE"], m["NCD"], m["M"], m["name"]))
+    lines.append("};")
+    lines.append("static const int intc_n_class = %d;" % len(metas))
+    tab = os.path.join(outdir, "class_table.inc")
+    src = "\n".join(lines) + "\n"
+    old = open(tab).read() if os.path.exists(tab) else None
+    if old != src:
+        with open(tab, "w") as fh:
+            fh.write(src)
+    return written, metas

@@ -1,0 +1,684 @@
+// intc_runtime.hip -- C-ABI of the MI355X contracted-ERI library (libintc.so).
+//
+// Entry points (declared in include/intc.h):
+//   * per-quartet twins of the reference's generated functions, same argument
+//     layout: intc_eri_cont_{sph,cart}_main and ..._work_array_size
+//   * batched path: a plan (canonical class bins, shell-pair list) built once
+//     from host index arrays, then executed on device-resident primitive data:
+//     pair builder kernel + one kernel per class bin.
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "intc_common.h"
+#include "boys_tables.inc"
+#include "gen/class_table.inc"
+
+#define INTC_API extern "C" __attribute__((visibility("default")))
+
+namespace {
+
+thread_local std::string g_err;
+int g_boys_mode = 0;
+
+int fail(int code, const std::string &msg)
+{
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(x)                                                                    \
+    do {                                                                             \
+        hipError_t e_ = (x);                                                         \
+        if (e_ != hipSuccess)                                                        \
+            return fail(-3, std::string(#x) + ": " + hipGetErrorString(e_));         \
+    } while (0)
+
+struct DevTables {
+    double *cheb = nullptr, *small = nullptr, *medium = nullptr;
+};
+std::mutex g_mu;
+std::vector<DevTables> g_tables;  // per device
+
+int ensure_tables(int dev, DevTables **out)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    if ((int)g_tables.size() <= dev) g_tables.resize(dev + 1);
+    DevTables &t = g_tables[dev];
+    if (!t.cheb) {
+        HIPCHK(hipMalloc(&t.cheb, sizeof(intc_boys_cheb_host)));
+        HIPCHK(hipMalloc(&t.small, sizeof(intc_boys_smallx_host)));
+        HIPCHK(hipMalloc(&t.medium, sizeof(intc_boys_mediumx_host)));
+        HIPCHK(hipMemcpy(t.cheb, intc_boys_cheb_host, sizeof(intc_boys_cheb_host), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(t.small, intc_boys_smallx_host, sizeof(intc_boys_smallx_host), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(t.medium, intc_boys_mediumx_host, sizeof(intc_boys_mediumx_host), hipMemcpyHostToDevice));
+    }
+    *out = &t;
+    return 0;
+}
+
+inline int ncc(int l) { return (l + 1) * (l + 2) / 2; }
+inline int ncc0(int l) { return l < 0 ? 0 : (l + 1) * (l + 2) * (l + 3) / 6; }
+inline int nsph(int l) { return 2 * l + 1; }
+
+const IntcClassEntry *find_class(int la, int lb, int lc, int ld, int sph)
+{
+    for (int i = 0; i < intc_n_class; ++i) {
+        const IntcClassEntry &c = intc_class_table[i];
+        if (c.la == la && c.lb == lb && c.lc == lc && c.ld == ld && c.sph == sph) return &c;
+    }
+    return nullptr;
+}
+
+// canonical order: la >= lb, lc >= ld, smaller pair (by (L, la)) in the bra.
+// perm[k] = original position of canonical position k.
+void canonical(const int l[4], int cls[4], int perm[4])
+{
+    int b0 = 0, b1 = 1, k0 = 2, k1 = 3;
+    if (l[b0] < l[b1]) std::swap(b0, b1);
+    if (l[k0] < l[k1]) std::swap(k0, k1);
+    int kb0 = l[b0] + l[b1], kb1 = l[b0], kk0 = l[k0] + l[k1], kk1 = l[k0];
+    if (kb0 > kk0 || (kb0 == kk0 && kb1 > kk1)) {
+        std::swap(b0, k0);
+        std::swap(b1, k1);
+    }
+    perm[0] = b0; perm[1] = b1; perm[2] = k0; perm[3] = k1;
+    for (int k = 0; k < 4; ++k) cls[k] = l[perm[k]];
+}
+
+// ---------------------------------------------------------------------------
+// pair builder: one thread per primitive pair
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) intc_build_pairs(
+    long long n_pp, const int *__restrict__ pp_pair, const int *__restrict__ pair_sa,
+    const int *__restrict__ pair_sb, const int *__restrict__ pair_off, const int *__restrict__ pair_nk,
+    const int *__restrict__ shell_off, const double *__restrict__ shell_cen,
+    const double *__restrict__ exps, const double *__restrict__ coefs,
+    double *__restrict__ pz, double *__restrict__ po2z, double *__restrict__ ppx, double *__restrict__ ppy,
+    double *__restrict__ ppz, double *__restrict__ pkap, double *__restrict__ pairA, double *__restrict__ pairAB)
+{
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n_pp) return;
+    const int pr = pp_pair[t];
+    const int sa = pair_sa[pr], sb = pair_sb[pr];
+    const int local = (int)(t - pair_off[pr]);
+    const int kb = pair_nk[pr];
+    const int ia = local / kb, ib = local - ia * kb;
+    const double al = exps[shell_off[sa] + ia], be = exps[shell_off[sb] + ib];
+    const double ca = coefs[shell_off[sa] + ia], cb = coefs[shell_off[sb] + ib];
+    const double Ax = shell_cen[3 * sa], Ay = shell_cen[3 * sa + 1], Az = shell_cen[3 * sa + 2];
+    const double Bx = shell_cen[3 * sb], By = shell_cen[3 * sb + 1], Bz = shell_cen[3 * sb + 2];
+    const double ABx = Ax - Bx, ABy = Ay - By, ABz = Az - Bz;
+    const double z = al + be, oz = 1.0 / z;
+    const double r2 = ABx * ABx + ABy * ABy + ABz * ABz;
+    pz[t] = z;
+    po2z[t] = 0.5 * oz;
+    ppx[t] = (al * Ax + be * Bx) * oz;
+    ppy[t] = (al * Ay + be * By) * oz;
+    ppz[t] = (al * Az + be * Bz) * oz;
+    // sqrt(2 pi^(5/2)) K_AB c_a c_b / zeta: the product of a bra and a ket
+    // kappa is the base prefactor 2 pi^(5/2) K_AB K_CD c_a c_b c_c c_d / (zeta eta)
+    pkap[t] = 5.914967172795612878 * exp(-al * be * oz * r2) * ca * cb * oz;
+    if (local == 0) {
+        pairA[3 * pr] = Ax; pairA[3 * pr + 1] = Ay; pairA[3 * pr + 2] = Az;
+        pairAB[3 * pr] = ABx; pairAB[3 * pr + 1] = ABy; pairAB[3 * pr + 2] = ABz;
+    }
+}
+
+template <class T>
+int dev_upload(T **d, const std::vector<T> &h)
+{
+    *d = nullptr;
+    if (h.empty()) return 0;
+    HIPCHK(hipMalloc(d, sizeof(T) * h.size()));
+    HIPCHK(hipMemcpy(*d, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice));
+    return 0;
+}
+
+}  // namespace
+
+struct IntcBin {
+    const IntcClassEntry *cls;
+    int q0, nq;
+    int team_groups;
+    int grid;
+    int block;
+};
+
+struct intc_plan {
+    int spherical = 0;
+    int device = 0;
+    long long n_quartet = 0;
+    long long n_pp = 0;
+    int n_pair = 0;
+    std::vector<IntcBin> bins;
+    // device index data
+    int *q_bra = nullptr, *q_ket = nullptr;
+    long long *q_off = nullptr;
+    unsigned long long *q_str = nullptr;
+    int *pair_sa = nullptr, *pair_sb = nullptr, *pair_off = nullptr, *pair_np = nullptr, *pair_nk = nullptr;
+    int *pp_pair = nullptr;
+    int *shell_off = nullptr;
+    // device pair data (filled by execute)
+    double *pz = nullptr, *po2z = nullptr, *ppx = nullptr, *ppy = nullptr, *ppz = nullptr, *pkap = nullptr;
+    double *pairA = nullptr, *pairAB = nullptr;
+    // host copies for tuning / inspection
+    std::vector<int> bin_prims;
+};
+
+namespace {
+
+void plan_free(intc_plan *p)
+{
+    void *ptrs[] = {p->q_bra, p->q_ket, p->q_off, p->q_str, p->pair_sa, p->pair_sb, p->pair_off,
+                    p->pair_np, p->pair_nk, p->pp_pair, p->shell_off, p->pz, p->po2z, p->ppx,
+                    p->ppy, p->ppz, p->pkap, p->pairA, p->pairAB};
+    for (void *q : ptrs)
+        if (q) (void)hipFree(q);
+}
+
+int default_team_groups(const IntcClassEntry *c, double mean_prims)
+{
+    // fill the wave with teams first; spend spare lanes on more groups per
+    // quartet when quartets have many primitive quartets
+    int G = c->G;
+    int tmax = 64 / G;
+    int T = 1;
+    while (T * 2 <= tmax && mean_prims >= 8.0 * T * 2) T *= 2;
+    return T;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C-ABI
+// ---------------------------------------------------------------------------
+INTC_API const char *intc_last_error(void) { return g_err.c_str(); }
+
+INTC_API int intc_version(void) { return 1; }
+
+INTC_API int intc_set_boys_mode(int mode)
+{
+    if (mode != 0 && mode != 1) return fail(-1, "boys mode must be 0 (Chebyshev) or 1 (reference scheme)");
+    g_boys_mode = mode;
+    return 0;
+}
+
+INTC_API int intc_init(void)
+{
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    DevTables *t;
+    return ensure_tables(dev, &t);
+}
+
+INTC_API int intc_class_supported(int la, int lb, int lc, int ld, int spherical)
+{
+    int l[4] = {la, lb, lc, ld}, cls[4], perm[4];
+    canonical(l, cls, perm);
+    return find_class(cls[0], cls[1], cls[2], cls[3], spherical ? 1 : 0) != nullptr;
+}
+
+INTC_API int intc_plan_create(intc_plan **out, int spherical, int n_shell, const int *shell_l,
+                              const int *shell_nprim, const int *shell_off, const double *shell_cen,
+                              long long n_quartet, const int *quartets, const long long *out_off)
+{
+    (void)shell_cen;
+    *out = nullptr;
+    if (n_shell < 0 || n_quartet < 0) return fail(-1, "negative size");
+    for (int s = 0; s < n_shell; ++s) {
+        if (shell_l[s] < 0 || shell_l[s] > 7) return fail(-1, "shell angular momentum out of range");
+        if (shell_nprim[s] < 1 || shell_nprim[s] > 64) return fail(-1, "shell primitive count out of range");
+    }
+    intc_plan *p = new intc_plan();
+    p->spherical = spherical ? 1 : 0;
+    HIPCHK(hipGetDevice(&p->device));
+    // canonical class and permutation per quartet
+    const int nq = (int)n_quartet;
+    std::vector<int> key(nq);
+    std::vector<const IntcClassEntry *> entries;
+    std::vector<int> cls_of(nq);
+    std::vector<int> prims(nq);
+    std::vector<std::array<int, 4>> perms(nq);
+    for (int q = 0; q < nq; ++q) {
+        int l[4], cls[4], perm[4];
+        for (int k = 0; k < 4; ++k) {
+            int s = quartets[4 * (size_t)q + k];
+            if (s < 0 || s >= n_shell) { plan_free(p); delete p; return fail(-1, "quartet shell id out of range"); }
+            l[k] = shell_l[s];
+        }
+        canonical(l, cls, perm);
+        const IntcClassEntry *e = find_class(cls[0], cls[1], cls[2], cls[3], p->spherical);
+        if (!e) {
+            plan_free(p);
+            delete p;
+            char buf[128];
+            snprintf(buf, sizeof buf, "class (%d %d|%d %d) %s not built into this library", l[0], l[1], l[2],
+                     l[3], spherical ? "spherical" : "cartesian");
+            return fail(-2, buf);
+        }
+        int idx = (int)(std::find(entries.begin(), entries.end(), e) - entries.begin());
+        if (idx == (int)entries.size()) entries.push_back(e);
+        cls_of[q] = idx;
+        for (int k = 0; k < 4; ++k) perms[q][k] = perm[k];
+        int pr = 1;
+        for (int k = 0; k < 4; ++k) pr *= shell_nprim[quartets[4 * (size_t)q + k]];
+        prims[q] = pr;
+    }
+    // order: by class (table order), then by primitive-quartet count descending
+    std::vector<int> order(nq);
+    for (int q = 0; q < nq; ++q) order[q] = q;
+    std::vector<int> tab_idx(entries.size());
+    for (size_t i = 0; i < entries.size(); ++i) tab_idx[i] = (int)(entries[i] - intc_class_table);
+    std::sort(order.begin(), order.end(), [&](int x, int y) {
+        int cx = tab_idx[cls_of[x]], cy = tab_idx[cls_of[y]];
+        if (cx != cy) return cx < cy;
+        if (prims[x] != prims[y]) return prims[x] > prims[y];
+        return x < y;
+    });
+    // pairs: one bra and one ket pair per quartet (canonical shell order)
+    std::vector<int> q_bra(nq), q_ket(nq), pair_sa(2 * (size_t)nq), pair_sb(2 * (size_t)nq);
+    std::vector<int> pair_off(2 * (size_t)nq), pair_np(2 * (size_t)nq), pair_nk(2 * (size_t)nq);
+    std::vector<long long> q_off(nq);
+    std::vector<unsigned long long> q_str(nq);
+    long long npp = 0;
+    for (int i = 0; i < nq; ++i) {
+        int q = order[i];
+        const int *s = quartets + 4 * (size_t)q;
+        const int *pm = perms[q].data();
+        int sh[4] = {s[pm[0]], s[pm[1]], s[pm[2]], s[pm[3]]};
+        for (int h = 0; h < 2; ++h) {
+            int pid = 2 * i + h;
+            int sa = sh[2 * h], sb = sh[2 * h + 1];
+            pair_sa[pid] = sa;
+            pair_sb[pid] = sb;
+            pair_off[pid] = (int)npp;
+            pair_np[pid] = shell_nprim[sa] * shell_nprim[sb];
+            pair_nk[pid] = shell_nprim[sb];
+            npp += pair_np[pid];
+        }
+        q_bra[i] = 2 * i;
+        q_ket[i] = 2 * i + 1;
+        q_off[i] = out_off[q];
+        // original row-major strides
+        int n[4];
+        for (int k = 0; k < 4; ++k) {
+            int l = shell_l[s[k]];
+            n[k] = p->spherical ? nsph(l) : ncc(l);
+        }
+        int st[4] = {n[1] * n[2] * n[3], n[2] * n[3], n[3], 1};
+        unsigned long long pk = 0;
+        for (int k = 0; k < 4; ++k) pk |= (unsigned long long)(st[pm[k]] & 0xffff) << (16 * k);
+        q_str[i] = pk;
+    }
+    if (npp > 0x7fffffffLL) { plan_free(p); delete p; return fail(-1, "too many primitive pairs for one plan"); }
+    std::vector<int> pp_pair(npp);
+    for (size_t pr = 0; pr < pair_off.size(); ++pr)
+        for (int k = 0; k < pair_np[pr]; ++k) pp_pair[pair_off[pr] + k] = (int)pr;
+    // bins
+    for (int i = 0; i < nq;) {
+        int j = i;
+        int c = cls_of[order[i]];
+        double sum = 0.0;
+        while (j < nq && cls_of[order[j]] == c) sum += prims[order[j++]];
+        IntcBin b;
+        b.cls = entries[c];
+        b.q0 = i;
+        b.nq = j - i;
+        b.team_groups = default_team_groups(b.cls, sum / (j - i));
+        b.block = 256;
+        int tpw = 64 / (b.cls->G * b.team_groups);
+        int teams_per_block = (b.block / 64) * tpw;
+        b.grid = std::min((b.nq + teams_per_block - 1) / teams_per_block, 256 * 32);
+        p->bins.push_back(b);
+        i = j;
+    }
+    std::vector<int> soff(shell_off, shell_off + n_shell);
+    int rc = 0;
+    rc |= dev_upload(&p->q_bra, q_bra);
+    rc |= dev_upload(&p->q_ket, q_ket);
+    rc |= dev_upload(&p->q_off, q_off);
+    rc |= dev_upload(&p->q_str, q_str);
+    rc |= dev_upload(&p->pair_sa, pair_sa);
+    rc |= dev_upload(&p->pair_sb, pair_sb);
+    rc |= dev_upload(&p->pair_off, pair_off);
+    rc |= dev_upload(&p->pair_np, pair_np);
+    rc |= dev_upload(&p->pair_nk, pair_nk);
+    rc |= dev_upload(&p->pp_pair, pp_pair);
+    rc |= dev_upload(&p->shell_off, soff);
+    if (rc) { plan_free(p); delete p; return rc; }
+    size_t npair = pair_sa.size();
+    p->n_pair = (int)npair;
+    p->n_pp = npp;
+    p->n_quartet = nq;
+    if (npp > 0) {
+        double **arrs[] = {&p->pz, &p->po2z, &p->ppx, &p->ppy, &p->ppz, &p->pkap};
+        for (double **a : arrs) {
+            hipError_t e = hipMalloc(a, sizeof(double) * npp);
+            if (e != hipSuccess) { plan_free(p); delete p; return fail(-3, "hipMalloc pair data"); }
+        }
+        if (hipMalloc(&p->pairA, sizeof(double) * 3 * npair) != hipSuccess ||
+            hipMalloc(&p->pairAB, sizeof(double) * 3 * npair) != hipSuccess) {
+            plan_free(p);
+            delete p;
+            return fail(-3, "hipMalloc pair geometry");
+        }
+    }
+    *out = p;
+    return 0;
+}
+
+INTC_API int intc_plan_num_bins(const intc_plan *p) { return (int)p->bins.size(); }
+
+INTC_API int intc_plan_bin_info(const intc_plan *p, int i, int *cls4, int *nq, int *team_groups, int *grid,
+                                int *G)
+{
+    if (i < 0 || i >= (int)p->bins.size()) return fail(-1, "bin index out of range");
+    const IntcBin &b = p->bins[i];
+    cls4[0] = b.cls->la; cls4[1] = b.cls->lb; cls4[2] = b.cls->lc; cls4[3] = b.cls->ld;
+    *nq = b.nq;
+    *team_groups = b.team_groups;
+    *grid = b.grid;
+    *G = b.cls->G;
+    return 0;
+}
+
+INTC_API int intc_plan_set_bin_tuning(intc_plan *p, int i, int team_groups, int grid)
+{
+    if (i < 0 || i >= (int)p->bins.size()) return fail(-1, "bin index out of range");
+    IntcBin &b = p->bins[i];
+    if (team_groups < 1 || team_groups * b.cls->G > 64) return fail(-1, "team_groups * G must be in [1, 64]");
+    if (grid < 1) return fail(-1, "grid must be positive");
+    b.team_groups = team_groups;
+    b.grid = grid;
+    return 0;
+}
+
+static int launch_bin(const intc_plan *p, const IntcBin &b, const IntcEriArgs &base, hipStream_t st)
+{
+    IntcEriArgs a = base;
+    a.q_bra = p->q_bra + b.q0;
+    a.q_ket = p->q_ket + b.q0;
+    a.q_off = p->q_off + b.q0;
+    a.q_str = p->q_str + b.q0;
+    a.nq = b.nq;
+    a.team_groups = b.team_groups;
+    int tpw = 64 / (b.cls->G * b.team_groups);
+    size_t lds = (size_t)(b.block / 64) * tpw * b.cls->NE * b.cls->NCD * sizeof(double);
+    hipError_t e = b.cls->fn(&a, b.grid, b.block, lds, st);
+    if (e != hipSuccess) return fail(-3, std::string("class kernel launch: ") + hipGetErrorString(e));
+    return 0;
+}
+
+static int fill_args(const intc_plan *p, const double *sph, double *out, IntcEriArgs *a)
+{
+    DevTables *t;
+    int rc = ensure_tables(p->device, &t);
+    if (rc) return rc;
+    memset(a, 0, sizeof(*a));
+    a->boys_mode = g_boys_mode;
+    a->pair_off = p->pair_off;
+    a->pair_np = p->pair_np;
+    a->pair_nk = p->pair_nk;
+    a->pair_A = p->pairA;
+    a->pair_AB = p->pairAB;
+    a->pp_zeta = p->pz;
+    a->pp_o2z = p->po2z;
+    a->pp_px = p->ppx;
+    a->pp_py = p->ppy;
+    a->pp_pz = p->ppz;
+    a->pp_kappa = p->pkap;
+    a->sph = sph;
+    a->boys_cheb = t->cheb;
+    a->boys_small = t->small;
+    a->boys_medium = t->medium;
+    a->out = out;
+    return 0;
+}
+
+INTC_API int intc_plan_build_pairs(intc_plan *p, const double *exps, const double *coefs, const double *shell_cen,
+                                   void *stream)
+{
+    if (p->n_pp == 0) return 0;
+    hipStream_t st = (hipStream_t)stream;
+    int blocks = (int)((p->n_pp + 255) / 256);
+    hipLaunchKernelGGL(intc_build_pairs, dim3(blocks), dim3(256), 0, st, p->n_pp, p->pp_pair, p->pair_sa,
+                       p->pair_sb, p->pair_off, p->pair_nk, p->shell_off, shell_cen, exps, coefs, p->pz, p->po2z,
+                       p->ppx, p->ppy, p->ppz, p->pkap, p->pairA, p->pairAB);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+INTC_API int intc_plan_run_bin(intc_plan *p, int i, const double *sph, double *out, void *stream)
+{
+    if (i < 0 || i >= (int)p->bins.size()) return fail(-1, "bin index out of range");
+    if (p->spherical && !sph) return fail(-1, "spherical plan needs the S matrices");
+    IntcEriArgs a;
+    int rc = fill_args(p, sph, out, &a);
+    if (rc) return rc;
+    return launch_bin(p, p->bins[i], a, (hipStream_t)stream);
+}
+
+INTC_API int intc_plan_execute(intc_plan *p, const double *exps, const double *coefs, const double *shell_cen,
+                               const double *sph, double *out, void *stream)
+{
+    if (p->spherical && !sph) return fail(-1, "spherical plan needs the S matrices");
+    int rc = intc_plan_build_pairs(p, exps, coefs, shell_cen, stream);
+    if (rc) return rc;
+    IntcEriArgs a;
+    rc = fill_args(p, sph, out, &a);
+    if (rc) return rc;
+    for (const IntcBin &b : p->bins) {
+        rc = launch_bin(p, b, a, (hipStream_t)stream);
+        if (rc) return rc;
+    }
+    return 0;
+}
+
+INTC_API int intc_plan_destroy(intc_plan *p)
+{
+    if (!p) return 0;
+    plan_free(p);
+    delete p;
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// per-quartet twins of the generated functions
+// ---------------------------------------------------------------------------
+static long hrr_layer(int la, int lb)
+{
+    long best = 0;
+    for (int lq = 0; lq <= la; ++lq) {
+        long s = (long)(ncc0(la + lb - lq) - ncc0(lb - 1)) * ncc(lq);
+        best = std::max(best, s);
+    }
+    return best;
+}
+
+static int work_size(int spherical, int la, int lb, int lc, int ld, int na, int nb, int nc, int nd, int ka, int kb,
+                     int kc, int kd)
+{
+    int L = la + lb, M = L + lc + ld;
+    long nvrr = (long)ncc0(L) * ncc0(lc) * ncc0(ld);
+    long nprim = (long)na * nb * nc * nd;
+    long mem = 2 * (nvrr * nprim + M + 1);
+    int np[4] = {na, nb, nc, nd}, nk[4] = {ka, kb, kc, kd};
+    long nmax = nprim;
+    for (int dir = 0; dir < 2; ++dir) {
+        int cur[4] = {np[0], np[1], np[2], np[3]};
+        for (int s = 0; s < 4; ++s) {
+            int k = dir == 0 ? s : 3 - s;
+            cur[k] = nk[k];
+            nmax = std::max(nmax, (long)cur[0] * cur[1] * cur[2] * cur[3]);
+        }
+    }
+    mem = std::max(mem, 2 * nvrr * nmax);
+    long ncont = (long)ka * kb * kc * kd;
+    int nc_c = spherical ? nsph(lc) : ncc(lc), nc_d = spherical ? nsph(ld) : ncc(ld);
+    mem = std::max(mem, 2 * hrr_layer(lb, la) * nc_c * nc_d * ncont);
+    long late = spherical ? 2L * ncc(la) * ncc(lb) * ncc(lc) * ncc(ld) * ncont
+                          : 2L * ncc(la) * ncc(lb) * nc_c * nc_d * ncont;
+    mem = std::max(mem, late);
+    return (int)mem;
+}
+
+INTC_API int intc_eri_cont_sph_work_array_size(int la, int lb, int lc, int ld, int na_prim, int nb_prim,
+                                               int nc_prim, int nd_prim, int na_cont, int nb_cont, int nc_cont,
+                                               int nd_cont)
+{
+    return work_size(1, la, lb, lc, ld, na_prim, nb_prim, nc_prim, nd_prim, na_cont, nb_cont, nc_cont, nd_cont);
+}
+
+INTC_API int intc_eri_cont_cart_work_array_size(int la, int lb, int lc, int ld, int na_prim, int nb_prim,
+                                                int nc_prim, int nd_prim, int na_cont, int nb_cont, int nc_cont,
+                                                int nd_cont)
+{
+    return work_size(0, la, lb, lc, ld, na_prim, nb_prim, nc_prim, nd_prim, na_cont, nb_cont, nc_cont, nd_cont);
+}
+
+struct ShellArg {
+    const double *x, *c;
+    int l, n, iskip, n_prim, n_cont, iskip_cont;
+    const double *contract, *sph;
+};
+
+// One contracted quartet (all contraction combinations) through the batched
+// device path: upload, plan of ncont_a*..*ncont_d quartets, run, scatter.
+static int per_quartet(int spherical, const ShellArg sa[4], double *output)
+{
+    int rc = 0;
+    int nk = 1;
+    for (int k = 0; k < 4; ++k) {
+        if (sa[k].l < 0 || sa[k].l > 7 || sa[k].n_prim < 1 || sa[k].n_cont < 1)
+            return fail(-1, "invalid shell arguments");
+        int want = spherical ? nsph(sa[k].l) : ncc(sa[k].l);
+        if (sa[k].n != want) return fail(-1, "component count n does not match l");
+        nk *= sa[k].n_cont;
+    }
+    // shells: one per (index, contraction) with that contraction's column
+    std::vector<int> sl, snp, soff;
+    std::vector<double> cen, ex, co;
+    int base[4];
+    for (int k = 0; k < 4; ++k) {
+        base[k] = (int)sl.size();
+        for (int j = 0; j < sa[k].n_cont; ++j) {
+            sl.push_back(sa[k].l);
+            snp.push_back(sa[k].n_prim);
+            soff.push_back((int)ex.size());
+            for (int i = 0; i < 3; ++i) cen.push_back(sa[k].c[i]);
+            for (int i = 0; i < sa[k].n_prim; ++i) {
+                ex.push_back(sa[k].x[i]);
+                co.push_back(sa[k].contract[i + j * sa[k].n_prim]);
+            }
+        }
+    }
+    int nblk = 1;
+    for (int k = 0; k < 4; ++k) nblk *= sa[k].n;
+    std::vector<int> quart;
+    std::vector<long long> off;
+    std::vector<int> combo;
+    for (int ka = 0; ka < sa[0].n_cont; ++ka)
+        for (int kb = 0; kb < sa[1].n_cont; ++kb)
+            for (int kc = 0; kc < sa[2].n_cont; ++kc)
+                for (int kd = 0; kd < sa[3].n_cont; ++kd) {
+                    quart.push_back(base[0] + ka);
+                    quart.push_back(base[1] + kb);
+                    quart.push_back(base[2] + kc);
+                    quart.push_back(base[3] + kd);
+                    off.push_back((long long)(off.size()) * nblk);
+                    combo.push_back(ka); combo.push_back(kb); combo.push_back(kc); combo.push_back(kd);
+                }
+    intc_plan *p = nullptr;
+    rc = intc_plan_create(&p, spherical, (int)sl.size(), sl.data(), snp.data(), soff.data(), cen.data(),
+                          (long long)nk, quart.data(), off.data());
+    if (rc) return rc;
+    double *d_ex = nullptr, *d_co = nullptr, *d_cen = nullptr, *d_sph = nullptr, *d_out = nullptr;
+    std::vector<double> sph_all;
+    if (spherical) {
+        sph_all.assign(intc_sph_off(8), 0.0);
+        for (int k = 0; k < 4; ++k) {
+            int l = sa[k].l;
+            for (int i = 0; i < ncc(l) * nsph(l); ++i) sph_all[intc_sph_off(l) + i] = sa[k].sph[i];
+        }
+    }
+    std::vector<double> res((size_t)nk * nblk);
+    auto cleanup = [&]() {
+        if (d_ex) (void)hipFree(d_ex);
+        if (d_co) (void)hipFree(d_co);
+        if (d_cen) (void)hipFree(d_cen);
+        if (d_sph) (void)hipFree(d_sph);
+        if (d_out) (void)hipFree(d_out);
+        intc_plan_destroy(p);
+    };
+#define PQCHK(x)                                                                       \
+    do {                                                                               \
+        hipError_t e_ = (x);                                                           \
+        if (e_ != hipSuccess) { cleanup(); return fail(-3, std::string(#x) + ": " + hipGetErrorString(e_)); } \
+    } while (0)
+    PQCHK(hipMalloc(&d_ex, sizeof(double) * ex.size()));
+    PQCHK(hipMalloc(&d_co, sizeof(double) * co.size()));
+    PQCHK(hipMalloc(&d_cen, sizeof(double) * cen.size()));
+    PQCHK(hipMalloc(&d_out, sizeof(double) * res.size()));
+    PQCHK(hipMemcpy(d_ex, ex.data(), sizeof(double) * ex.size(), hipMemcpyHostToDevice));
+    PQCHK(hipMemcpy(d_co, co.data(), sizeof(double) * co.size(), hipMemcpyHostToDevice));
+    PQCHK(hipMemcpy(d_cen, cen.data(), sizeof(double) * cen.size(), hipMemcpyHostToDevice));
+    if (spherical) {
+        PQCHK(hipMalloc(&d_sph, sizeof(double) * sph_all.size()));
+        PQCHK(hipMemcpy(d_sph, sph_all.data(), sizeof(double) * sph_all.size(), hipMemcpyHostToDevice));
+    }
+    rc = intc_plan_execute(p, d_ex, d_co, d_cen, d_sph, d_out, nullptr);
+    if (rc) { cleanup(); return rc; }
+    PQCHK(hipDeviceSynchronize());
+    PQCHK(hipMemcpy(res.data(), d_out, sizeof(double) * res.size(), hipMemcpyDeviceToHost));
+#undef PQCHK
+    cleanup();
+    // scatter: out[sum_X(iX*iskipX + icontX*iskipX_cont)]
+    for (int k = 0; k < nk; ++k) {
+        const int *cb = &combo[4 * k];
+        const double *blk = &res[(size_t)k * nblk];
+        size_t i = 0;
+        for (int a = 0; a < sa[0].n; ++a)
+            for (int b = 0; b < sa[1].n; ++b)
+                for (int c = 0; c < sa[2].n; ++c)
+                    for (int d = 0; d < sa[3].n; ++d, ++i)
+                        output[(long)a * sa[0].iskip + (long)cb[0] * sa[0].iskip_cont + (long)b * sa[1].iskip +
+                               (long)cb[1] * sa[1].iskip_cont + (long)c * sa[2].iskip +
+                               (long)cb[2] * sa[2].iskip_cont + (long)d * sa[3].iskip +
+                               (long)cb[3] * sa[3].iskip_cont] = blk[i];
+    }
+    return 0;
+}
+
+#define INTC_SHELL_PARAMS(X)                                                                        \
+    const double *x##X, const double c##X[3], const int l##X, const int n##X, const int iskip##X,   \
+        const int n##X##_prim, const int n##X##_cont, const int iskip##X##_cont,                    \
+        const double *contract_array_##X
+
+#define INTC_MK(X, S) {x##X, c##X, l##X, n##X, iskip##X, n##X##_prim, n##X##_cont, iskip##X##_cont, contract_array_##X, S}
+
+INTC_API int intc_eri_cont_sph_main(INTC_SHELL_PARAMS(a), const double *sph_trans_array_a, INTC_SHELL_PARAMS(b),
+                                    const double *sph_trans_array_b, INTC_SHELL_PARAMS(c),
+                                    const double *sph_trans_array_c, INTC_SHELL_PARAMS(d),
+                                    const double *sph_trans_array_d, double *work_array, double *output_array)
+{
+    (void)work_array;
+    ShellArg s[4] = {INTC_MK(a, sph_trans_array_a), INTC_MK(b, sph_trans_array_b), INTC_MK(c, sph_trans_array_c),
+                     INTC_MK(d, sph_trans_array_d)};
+    return per_quartet(1, s, output_array);
+}
+
+INTC_API int intc_eri_cont_cart_main(INTC_SHELL_PARAMS(a), INTC_SHELL_PARAMS(b), INTC_SHELL_PARAMS(c),
+                                     INTC_SHELL_PARAMS(d), double *work_array, double *output_array)
+{
+    (void)work_array;
+    ShellArg s[4] = {INTC_MK(a, nullptr), INTC_MK(b, nullptr), INTC_MK(c, nullptr), INTC_MK(d, nullptr)};
+    return per_quartet(0, s, output_array);
+}

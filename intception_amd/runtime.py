@@ -1,0 +1,128 @@
+"""Host side of the batched ERI path: device buffers (torch), plans, launches.
+
+``EriPlan(batch)`` bins a ``basis.QuartetBatch`` by canonical class inside
+libintc (C++), uploads the primitive data and S matrices, and ``run()`` executes
+the pair builder and the class kernels on the current HIP stream.  Everything
+numeric happens in libintc's kernels; this module only moves memory.
+"""
+import ctypes
+
+import numpy as np
+
+from . import lib as _lib
+from .basis import SPH, ncc, nsph
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def sph_blob(lmax=7):
+    """S_l for l = 0..lmax concatenated column-major (the device layout)."""
+    parts = []
+    for l in range(lmax + 1):
+        S = SPH[l] if l in SPH else None
+        if S is None:
+            from .basis import sph_matrix
+            S = sph_matrix(l)
+        parts.append(np.asfortranarray(S).reshape(-1, order="F"))
+    return np.concatenate(parts)
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+class EriPlan:
+    def __init__(self, batch, device="cuda"):
+        torch = _torch()
+        L = _lib.load()
+        self.batch = batch
+        self.device = torch.device(device)
+        self._L = L
+        b = batch
+        sl = np.ascontiguousarray(b.shell_l, dtype=np.int32)
+        snp = np.ascontiguousarray(b.shell_nprim, dtype=np.int32)
+        soff = np.ascontiguousarray(b.shell_off, dtype=np.int32)
+        cen = np.ascontiguousarray(b.shell_cen, dtype=np.float64)
+        q = np.ascontiguousarray(b.quartets.reshape(-1), dtype=np.int32)
+        oo = np.ascontiguousarray(b.out_off, dtype=np.int64)
+        h = ctypes.c_void_p()
+        ip = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+        with torch.cuda.device(self.device):
+            _lib.check(L.intc_plan_create(ctypes.byref(h), int(b.spherical), len(sl), ip(sl), ip(snp), ip(soff),
+                                          cen.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), len(oo), ip(q),
+                                          oo.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong))))
+        self.handle = h
+        self.d_exps = torch.from_numpy(np.ascontiguousarray(b.exps, dtype=np.float64)).to(self.device)
+        self.d_coefs = torch.from_numpy(np.ascontiguousarray(b.coefs, dtype=np.float64)).to(self.device)
+        self.d_cen = torch.from_numpy(cen.reshape(-1)).to(self.device)
+        self.d_sph = torch.from_numpy(sph_blob()).to(self.device) if b.spherical else None
+        self.out_size = int(b.out_size)
+
+    def bins(self):
+        L = self._L
+        n = L.intc_plan_num_bins(self.handle)
+        out = []
+        for i in range(n):
+            cls = (ctypes.c_int * 4)()
+            nq, T, grid, G = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+            _lib.check(L.intc_plan_bin_info(self.handle, i, cls, ctypes.byref(nq), ctypes.byref(T),
+                                            ctypes.byref(grid), ctypes.byref(G)))
+            out.append({"bin": i, "cls": tuple(cls), "nq": nq.value, "team_groups": T.value,
+                        "grid": grid.value, "G": G.value})
+        return out
+
+    def set_tuning(self, i, team_groups, grid):
+        _lib.check(self._L.intc_plan_set_bin_tuning(self.handle, int(i), int(team_groups), int(grid)))
+
+    def _stream(self):
+        torch = _torch()
+        return ctypes.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
+
+    def alloc_out(self):
+        torch = _torch()
+        return torch.empty(self.out_size, dtype=torch.float64, device=self.device)
+
+    def build_pairs(self):
+        _lib.check(self._L.intc_plan_build_pairs(self.handle, _ptr(self.d_exps), _ptr(self.d_coefs),
+                                                 _ptr(self.d_cen), self._stream()))
+
+    def run_bin(self, i, out):
+        sph = _ptr(self.d_sph) if self.d_sph is not None else ctypes.c_void_p()
+        _lib.check(self._L.intc_plan_run_bin(self.handle, int(i), sph, _ptr(out), self._stream()))
+
+    def run(self, out=None):
+        if out is None:
+            out = self.alloc_out()
+        sph = _ptr(self.d_sph) if self.d_sph is not None else ctypes.c_void_p()
+        _lib.check(self._L.intc_plan_execute(self.handle, _ptr(self.d_exps), _ptr(self.d_coefs),
+                                             _ptr(self.d_cen), sph, _ptr(out), self._stream()))
+        return out
+
+    def close(self):
+        if self.handle:
+            self._L.intc_plan_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def set_boys_mode(mode):
+    _lib.check(_lib.load().intc_set_boys_mode(int(mode)))
+
+
+def compute(batch, device="cuda"):
+    """Convenience: all integrals of a batch as a host numpy array."""
+    torch = _torch()
+    plan = EriPlan(batch, device)
+    out = plan.run()
+    torch.cuda.synchronize(plan.device)
+    res = out.cpu().numpy()
+    plan.close()
+    return res
