@@ -1,0 +1,187 @@
+"""Benchmark: contracted ERIs/s on one MI355X.
+
+Workload (default): 1,000,000 contracted shell quartets, class drawn uniformly
+over (la lb|lc ld) with l in {s,p,d} (81 classes), contraction depth per shell
+uniform in 1..6, centres uniform in a 10-bohr cube, exponents log-uniform in
+[0.1, 100], spherical output, FP64.  A step is one pass of the whole batch:
+device pair builder + one kernel per canonical class bin, inputs resident in
+HBM.  Host-side planning (class binning) happens once, before the timed region.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu]
+
+Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 (vector and matrix)
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(batch, n_sample, seed=7):
+    """The oracle (C restatement of the reference's generated code, OpenMP)
+    on a random sample of the same workload, on this host's cores."""
+    from intception_amd import basis
+    from oracle import oracle
+    rng = np.random.default_rng(seed)
+    idx = np.sort(rng.choice(batch.n_quartet, size=min(n_sample, batch.n_quartet), replace=False))
+    sub = batch.subset(idx)
+    threads = min(16, os.cpu_count() or 1)
+    oracle.set_boys_mode(oracle.BOYS_REFERENCE)
+    oracle.eri_batch(sub.subset(np.arange(min(500, sub.n_quartet))), sub.spherical, basis.SPH, threads)
+    t0 = time.perf_counter()
+    oracle.eri_batch(sub, sub.spherical, basis.SPH, threads)
+    dt = time.perf_counter() - t0
+    return {"value": float(sub.out_size / dt), "unit": "integrals/s", "cores": threads, "kind": "port",
+            "sample": "%d random quartets of the same workload (%.1f s); oracle/eri_oracle.c, gcc -O3 "
+                      "-march=x86-64-v3 -fopenmp, schedule(dynamic,64)" % (sub.n_quartet, dt),
+            "quartets_per_s": float(sub.n_quartet / dt)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--quartets", type=int, default=1_000_000)
+    ap.add_argument("--cpu-sample", type=int, default=150_000)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--check", type=int, default=3000, help="quartets checked against the oracle")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: quartets shard across GPUs without exchange "
+              "(see the sharding design); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from intception_amd import basis, flops_model, lib, runtime
+
+    t0 = time.perf_counter()
+    batch = basis.config1(n_quartet=args.quartets)
+    log("workload built: %d quartets, %d integrals (%.1fs)" % (batch.n_quartet, batch.out_size,
+                                                                time.perf_counter() - t0))
+    lib.check(lib.load().intc_init())
+    t0 = time.perf_counter()
+    plan = runtime.EriPlan(batch)
+    t_plan = time.perf_counter() - t0
+    out = plan.alloc_out()
+    dev = plan.device
+    for _ in range(args.warmup):
+        plan.run(out)
+    torch.cuda.synchronize(dev)
+    stream = torch.cuda.current_stream(dev)
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize(dev)
+    t_wall = time.perf_counter()
+    ev0.record(stream)
+    for _ in range(args.steps):
+        plan.run(out)
+    ev1.record(stream)
+    torch.cuda.synchronize(dev)
+    t_wall = time.perf_counter() - t_wall
+    ms_step = ev0.elapsed_time(ev1) / args.steps
+    log("timed: %d steps, %.3f ms/step (wall %.3f ms/step)" % (args.steps, ms_step, 1e3 * t_wall / args.steps))
+
+    # per-kernel durations (events on the launch stream) for the roofline
+    bins = plan.bins()
+    flops_q = flops_model.batch_flops(batch)
+    classes = batch.classes()
+    from intception_amd.codegen import canonical
+    canon = np.array([canonical(*c)[0] for c in map(tuple, classes)])
+    reps = max(3, args.steps)
+    per_bin = []
+    ev_a, ev_b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev_a.record(stream)
+    for _ in range(reps):
+        plan.build_pairs()
+    ev_b.record(stream)
+    torch.cuda.synchronize(dev)
+    t_pairs = ev_a.elapsed_time(ev_b) / reps
+    for b in bins:
+        ev_a.record(stream)
+        for _ in range(reps):
+            plan.run_bin(b["bin"], out)
+        ev_b.record(stream)
+        torch.cuda.synchronize(dev)
+        ms = ev_a.elapsed_time(ev_b) / reps
+        sel = np.all(canon == np.array(b["cls"]), axis=1)
+        per_bin.append(dict(b, ms=ms, flops=float(flops_q[sel].sum()), integrals=int(batch.sizes()[sel].sum())))
+    dom = max(per_bin, key=lambda r: r["ms"])
+    achieved = dom["flops"] / (dom["ms"] * 1e-3) / 1e12
+    total_flops = float(flops_q.sum())
+
+    # parity spot check against the oracle (Chebyshev Boys vs accurate Boys)
+    check = {}
+    if args.check > 0:
+        from oracle import oracle
+        idx = np.arange(0, batch.n_quartet, max(1, batch.n_quartet // args.check))[:args.check]
+        sub = batch.subset(idx)
+        oracle.set_boys_mode(oracle.BOYS_EXACT)
+        ref = oracle.eri_batch(sub, True, basis.SPH, min(16, os.cpu_count() or 1))
+        host = out.cpu().numpy()
+        sz = batch.sizes()
+        got = np.concatenate([host[batch.out_off[q]:batch.out_off[q] + sz[q]] for q in idx])
+        err = np.abs(got - ref)
+        check = {"quartets": int(len(idx)), "max_abs_err": float(err.max()),
+                 "max_rel_err": float((err / np.maximum(np.abs(ref), 1e-300))[np.abs(ref) > 1e-10].max()),
+                 "within_1e-12_rel_1e-14_abs": bool(np.all(err <= 1e-14 + 1e-12 * np.abs(ref)))}
+        log("parity check:", check)
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(batch, args.cpu_sample)
+        log("cpu baseline:", cpu)
+
+    ints_per_s = batch.out_size / (ms_step * 1e-3)
+    res = {
+        "metric": "contracted ERIs/s on 1 MI355X (FP64 plain path)",
+        "value": float(ints_per_s),
+        "unit": "integrals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": float(ms_step),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "config1: %d quartets, 81 classes (l<=2 per shell), K 1..6, spherical, "
+                               "FP64" % batch.n_quartet,
+                   "quartets": int(batch.n_quartet), "integrals": int(batch.out_size),
+                   "primitive_quartets": int(batch.shell_nprim[batch.quartets].astype(np.int64).prod(1).sum()),
+                   "seed": 202601},
+        "quartets_per_s": float(batch.n_quartet / (ms_step * 1e-3)),
+        "roofline": {"bound": "mfma", "achieved": float(achieved), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": float(achieved / FP64_PEAK_TFLOPS), "traffic": None,
+                     "kernel": "intc_eri_d%d%d%d%d_s" % tuple(dom["cls"]), "kernel_ms": dom["ms"],
+                     "kernel_flops": dom["flops"],
+                     "note": "FP64 VALU-bound; peak = MI355X FP64 (vector = matrix = 78.6 TF/s); flops = "
+                             "reference-scheme algorithmic count (intception_amd/flops_model.py)",
+                     "job_achieved": total_flops / (ms_step * 1e-3) / 1e12,
+                     "job_frac": total_flops / (ms_step * 1e-3) / 1e12 / FP64_PEAK_TFLOPS},
+        "cpu_baseline": cpu,
+        "parity": check,
+        "plan_s": t_plan,
+        "pair_build_ms": t_pairs,
+        "bins": [{"cls": r["cls"], "nq": r["nq"], "T": r["team_groups"], "ms": round(r["ms"], 4),
+                  "tflops": round(r["flops"] / (r["ms"] * 1e-3) / 1e12, 3)} for r in per_bin],
+    }
+    print(json.dumps(res), flush=True)
+    plan.close()
+
+
+if __name__ == "__main__":
+    main()
