@@ -58,6 +58,9 @@ def main():
     ap.add_argument("--quartets", type=int, default=1_000_000)
     ap.add_argument("--cpu-sample", type=int, default=150_000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--streams", type=int, default=4)
+    ap.add_argument("--tune-out", default=None, help="also write the tuning table here")
     ap.add_argument("--check", type=int, default=3000, help="quartets checked against the oracle")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -78,21 +81,40 @@ def main():
     t_plan = time.perf_counter() - t0
     out = plan.alloc_out()
     dev = plan.device
+    stream = torch.cuda.current_stream(dev)
+
+    def timed(fn, n):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize(dev)
+        e0.record(stream)
+        for _ in range(n):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize(dev)
+        return e0.elapsed_time(e1) / n
+
     for _ in range(args.warmup):
         plan.run(out)
-    torch.cuda.synchronize(dev)
-    stream = torch.cuda.current_stream(dev)
-    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ms_default = timed(lambda: plan.run(out), 2)
+    log("one launch per class, heuristic shapes: %.3f ms/step" % ms_default)
+    from intception_amd import autotune
+    n_cached = autotune.apply_cache(plan)
+    log("tuning cache: %d of %d bins" % (n_cached, len(plan.bins())))
+    if not args.no_tune:
+        t0 = time.perf_counter()
+        autotune.tune(plan, out, reps=2, log=log, save_path=args.tune_out)
+        log("autotune: %.1f s" % (time.perf_counter() - t0))
+    ms_eager = timed(lambda: plan.run(out), 2)
+    log("one launch per class, tuned shapes: %.3f ms/step" % ms_eager)
+    plan.capture(out, n_streams=args.streams)
+    for _ in range(args.warmup):
+        plan.replay()
     torch.cuda.synchronize(dev)
     t_wall = time.perf_counter()
-    ev0.record(stream)
-    for _ in range(args.steps):
-        plan.run(out)
-    ev1.record(stream)
-    torch.cuda.synchronize(dev)
+    ms_step = timed(plan.replay, args.steps)
     t_wall = time.perf_counter() - t_wall
-    ms_step = ev0.elapsed_time(ev1) / args.steps
-    log("timed: %d steps, %.3f ms/step (wall %.3f ms/step)" % (args.steps, ms_step, 1e3 * t_wall / args.steps))
+    log("timed: %d graph replays (%d streams), %.3f ms/step (wall %.3f ms/step)"
+        % (args.steps, args.streams, ms_step, 1e3 * t_wall / args.steps))
 
     # per-kernel durations (events on the launch stream) for the roofline
     bins = plan.bins()
@@ -102,6 +124,7 @@ def main():
     canon = np.array([canonical(*c)[0] for c in map(tuple, classes)])
     reps = max(3, args.steps)
     per_bin = []
+    torch.cuda.synchronize(dev)
     ev_a, ev_b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     ev_a.record(stream)
     for _ in range(reps):
@@ -175,6 +198,9 @@ def main():
         "cpu_baseline": cpu,
         "parity": check,
         "plan_s": t_plan,
+        "ms_one_launch_per_class_heuristic": ms_default,
+        "ms_one_launch_per_class_tuned": ms_eager,
+        "graph_streams": args.streams,
         "pair_build_ms": t_pairs,
         "bins": [{"cls": r["cls"], "nq": r["nq"], "T": r["team_groups"], "ms": round(r["ms"], 4),
                   "tflops": round(r["flops"] / (r["ms"] * 1e-3) / 1e12, 3)} for r in per_bin],

@@ -92,6 +92,12 @@ int intc_plan_build_pairs(intc_plan *plan, const double *exps, const double *coe
 int intc_plan_run_bin(intc_plan *plan, int bin, const double *sph, double *out, void *stream);
 int intc_plan_execute(intc_plan *plan, const double *exps, const double *coefs, const double *shell_cen,
                       const double *sph, double *out, void *stream);
+/* One hipGraph holding the pair builder and every class-bin kernel; with
+ * n_streams > 1 the bins are independent nodes that run concurrently.  The
+ * graph binds the given device pointers; replay with intc_plan_graph_launch. */
+int intc_plan_graph_create(intc_plan *plan, const double *exps, const double *coefs, const double *shell_cen,
+                           const double *sph, double *out, int n_streams, void *stream);
+int intc_plan_graph_launch(intc_plan *plan, void *stream);
 int intc_plan_destroy(intc_plan *plan);
 
 #ifdef __cplusplus

@@ -219,7 +219,8 @@ def emit_class_source(cls, spherical):
         o("    {%s}," % ", ".join(str(v) for v in r))
     o("};")
     o("")
-    o('extern "C" __global__ void __launch_bounds__(256) %s(const IntcEriArgs a)' % name)
+    o("template <int BM>")
+    o("__device__ __forceinline__ void %s_body(const IntcEriArgs &a)" % name)
     o("{")
     o.ind = 1
     o("constexpr int G = %d, NE = %d, NCD = %d, M = %d;" % (G, NE, NCD, M))
@@ -266,7 +267,7 @@ def emit_class_source(cls, spherical):
     o("const double Tb = rho * (PQx * PQx + PQy * PQy + PQz * PQz);")
     o("const double pref = kab * kcd * sqrt(os);")
     o("double F[M + 1];")
-    o("intc_boys<M>(Tb, F, a);")
+    o("intc_boys<M, BM>(Tb, F, a);")
     if LX > 0:
         o("const double eo = eta * os;")
         o("const double PAx = Px - Ax, PAy = Py - Ay, PAz = Pz - Az;")
@@ -372,10 +373,17 @@ def emit_class_source(cls, spherical):
     o.ind = 0
     o("}")
     o("")
+    o('extern "C" __global__ void __launch_bounds__(256) %s(const IntcEriArgs a) { %s_body<0>(a); }'
+      % (name, name))
+    o('extern "C" __global__ void __launch_bounds__(256) %s_refboys(const IntcEriArgs a) { %s_body<1>(a); }'
+      % (name, name))
     o('extern "C" hipError_t %s_launch(const IntcEriArgs *a, int grid, int block, size_t lds, hipStream_t st)'
       % name)
     o("{")
-    o("    hipLaunchKernelGGL(%s, dim3(grid), dim3(block), lds, st, *a);" % name)
+    o("    if (a->boys_mode == 0)")
+    o("        hipLaunchKernelGGL(%s, dim3(grid), dim3(block), lds, st, *a);" % name)
+    o("    else")
+    o("        hipLaunchKernelGGL(%s_refboys, dim3(grid), dim3(block), lds, st, *a);" % name)
     o("    return hipGetLastError();")
     o("}")
     meta = dict(cls=cls, spherical=spherical, G=G, NE=NE, NCD=NCD, M=M, name=name)

@@ -69,10 +69,10 @@ __device__ __forceinline__ float intc_shflf(float v, int src4)
 // the exp(-T) term beyond.  Mode 1: the reference's three-branch scheme
 // (src/intception/generator/callables/generator_boys_function.py:98-205) on
 // grid tables of the same shape as the reference's.
-template <int M>
+template <int M, int BM>
 __device__ __forceinline__ void intc_boys(double T, double *F, const IntcEriArgs &a)
 {
-    if (a.boys_mode == 0) {
+    if (BM == 0) {
         if (T < (double)INTC_BOYS_NINT) {
             int k = (int)T;
             double t = 2.0 * (T - (double)k) - 1.0;

@@ -171,12 +171,26 @@ struct intc_plan {
     double *pairA = nullptr, *pairAB = nullptr;
     // host copies for tuning / inspection
     std::vector<int> bin_prims;
+    // captured execution: pair builder, then every class bin as an independent
+    // graph node (bins fan out over side streams during capture)
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    std::vector<hipStream_t> side;
+    std::vector<hipEvent_t> events;
 };
 
 namespace {
 
 void plan_free(intc_plan *p)
 {
+    if (p->graph_exec) (void)hipGraphExecDestroy(p->graph_exec);
+    if (p->graph) (void)hipGraphDestroy(p->graph);
+    for (hipEvent_t e : p->events) (void)hipEventDestroy(e);
+    for (hipStream_t st : p->side) (void)hipStreamDestroy(st);
+    p->graph_exec = nullptr;
+    p->graph = nullptr;
+    p->events.clear();
+    p->side.clear();
     void *ptrs[] = {p->q_bra, p->q_ket, p->q_off, p->q_str, p->pair_sa, p->pair_sb, p->pair_off,
                     p->pair_np, p->pair_nk, p->pp_pair, p->shell_off, p->pz, p->po2z, p->ppx,
                     p->ppy, p->ppz, p->pkap, p->pairA, p->pairAB};
@@ -184,15 +198,33 @@ void plan_free(intc_plan *p)
         if (q) (void)hipFree(q);
 }
 
-int default_team_groups(const IntcClassEntry *c, double mean_prims)
+// lanes of a wave doing useful work when teams of G*T lanes tile it
+double lane_use(int G, int T)
 {
-    // fill the wave with teams first; spend spare lanes on more groups per
-    // quartet when quartets have many primitive quartets
+    int tl = G * T;
+    return tl > 64 ? 0.0 : (double)((64 / tl) * tl) / 64.0;
+}
+
+// Team size: among the T that keep the wave's lane use within 10% of the
+// best, the smallest that still gives >= 8 waves per SIMD over the bin
+// (teams of one group finish their epilogue with every lane busy; larger
+// teams only pay off when a bin is too small to fill 1024 SIMDs).
+int default_team_groups(const IntcClassEntry *c, int nq, double mean_prims)
+{
     int G = c->G;
-    int tmax = 64 / G;
-    int T = 1;
-    while (T * 2 <= tmax && mean_prims >= 8.0 * T * 2) T *= 2;
-    return T;
+    double best = 0.0;
+    for (int T = 1; G * T <= 64; ++T) best = std::max(best, lane_use(G, T));
+    int pick = 1, last = 1;
+    for (int T = 1; G * T <= 64; ++T) {
+        if (lane_use(G, T) < 0.9 * best) continue;
+        if (T > 1 && mean_prims < 2.0 * T) break;
+        last = T;
+        double waves = (double)nq * G * T / 64.0;
+        if (waves >= 8.0 * 1024.0) { pick = T; break; }
+        pick = T;
+    }
+    (void)last;
+    return pick;
 }
 
 }  // namespace
@@ -332,7 +364,7 @@ INTC_API int intc_plan_create(intc_plan **out, int spherical, int n_shell, const
         b.cls = entries[c];
         b.q0 = i;
         b.nq = j - i;
-        b.team_groups = default_team_groups(b.cls, sum / (j - i));
+        b.team_groups = default_team_groups(b.cls, j - i, sum / (j - i));
         b.block = 256;
         int tpw = 64 / (b.cls->G * b.team_groups);
         int teams_per_block = (b.block / 64) * tpw;
@@ -479,6 +511,87 @@ INTC_API int intc_plan_execute(intc_plan *p, const double *exps, const double *c
         rc = launch_bin(p, b, a, (hipStream_t)stream);
         if (rc) return rc;
     }
+    return 0;
+}
+
+// Capture pair building and all class bins into one hipGraph.  With
+// n_streams > 1 the bins are spread over side streams forked from `stream`,
+// so the graph has one pair-builder node followed by independent bin nodes
+// that the device runs concurrently; n_streams = 1 keeps them in sequence.
+INTC_API int intc_plan_graph_create(intc_plan *p, const double *exps, const double *coefs, const double *shell_cen,
+                                    const double *sph, double *out, int n_streams, void *stream)
+{
+    if (n_streams < 1 || n_streams > 16) return fail(-1, "n_streams must be in [1, 16]");
+    if (p->spherical && !sph) return fail(-1, "spherical plan needs the S matrices");
+    if (p->graph_exec) {
+        (void)hipGraphExecDestroy(p->graph_exec);
+        p->graph_exec = nullptr;
+    }
+    if (p->graph) {
+        (void)hipGraphDestroy(p->graph);
+        p->graph = nullptr;
+    }
+    hipStream_t st = (hipStream_t)stream;
+    bool own = false;
+    if (!st) {
+        HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        own = true;
+    }
+    while ((int)p->side.size() < n_streams - 1) {
+        hipStream_t s2;
+        HIPCHK(hipStreamCreateWithFlags(&s2, hipStreamNonBlocking));
+        p->side.push_back(s2);
+    }
+    while ((int)p->events.size() < n_streams) {
+        hipEvent_t e;
+        HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        p->events.push_back(e);
+    }
+    IntcEriArgs a;
+    int rc = fill_args(p, sph, out, &a);
+    if (rc) return rc;
+    HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    rc = intc_plan_build_pairs(p, exps, coefs, shell_cen, st);
+    if (!rc && n_streams > 1) {
+        if (hipEventRecord(p->events[0], st) != hipSuccess) rc = fail(-3, "event record");
+        for (int k = 0; !rc && k < n_streams - 1; ++k)
+            if (hipStreamWaitEvent(p->side[k], p->events[0], 0) != hipSuccess) rc = fail(-3, "stream wait");
+    }
+    // largest bins first, round-robin over the streams
+    std::vector<int> ord(p->bins.size());
+    for (size_t i = 0; i < ord.size(); ++i) ord[i] = (int)i;
+    std::sort(ord.begin(), ord.end(), [&](int x, int y) {
+        return (double)p->bins[x].nq * p->bins[x].cls->NE * p->bins[x].cls->NCD >
+               (double)p->bins[y].nq * p->bins[y].cls->NE * p->bins[y].cls->NCD;
+    });
+    for (size_t k = 0; !rc && k < ord.size(); ++k) {
+        int sidx = (int)(k % n_streams);
+        hipStream_t s2 = sidx == 0 ? st : p->side[sidx - 1];
+        rc = launch_bin(p, p->bins[ord[k]], a, s2);
+    }
+    if (!rc && n_streams > 1) {
+        for (int k = 0; !rc && k < n_streams - 1; ++k) {
+            if (hipEventRecord(p->events[k + 1], p->side[k]) != hipSuccess) rc = fail(-3, "event record");
+            else if (hipStreamWaitEvent(st, p->events[k + 1], 0) != hipSuccess) rc = fail(-3, "stream wait");
+        }
+    }
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(st, &g);
+    if (own) (void)hipStreamDestroy(st);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) return fail(-3, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    p->graph = g;
+    HIPCHK(hipGraphInstantiate(&p->graph_exec, g, nullptr, nullptr, 0));
+    return 0;
+}
+
+INTC_API int intc_plan_graph_launch(intc_plan *p, void *stream)
+{
+    if (!p->graph_exec) return fail(-1, "no graph captured for this plan");
+    HIPCHK(hipGraphLaunch(p->graph_exec, (hipStream_t)stream));
     return 0;
 }
 

@@ -36,6 +36,9 @@ SIGNATURES = [
     ("intc_plan_build_pairs", c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     ("intc_plan_run_bin", c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
     ("intc_plan_execute", c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    ("intc_plan_graph_create", c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int,
+                                       c_void_p]),
+    ("intc_plan_graph_launch", c_int, [c_void_p, c_void_p]),
     ("intc_plan_destroy", c_int, [c_void_p]),
 ]
 

@@ -101,6 +101,19 @@ class EriPlan:
                                              _ptr(self.d_cen), sph, _ptr(out), self._stream()))
         return out
 
+    def capture(self, out, n_streams=4):
+        """Capture pair building + all bins into one hipGraph bound to `out`."""
+        sph = _ptr(self.d_sph) if self.d_sph is not None else ctypes.c_void_p()
+        _lib.check(self._L.intc_plan_graph_create(self.handle, _ptr(self.d_exps), _ptr(self.d_coefs),
+                                                  _ptr(self.d_cen), sph, _ptr(out), int(n_streams),
+                                                  self._stream()))
+        self._graph_out = out
+        return out
+
+    def replay(self):
+        _lib.check(self._L.intc_plan_graph_launch(self.handle, self._stream()))
+        return self._graph_out
+
     def close(self):
         if self.handle:
             self._L.intc_plan_destroy(self.handle)

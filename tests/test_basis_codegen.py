@@ -1,0 +1,102 @@
+"""CPU: caller-side data (normalisation, spherical matrices), the canonical
+class mapping, the generated kernels' lane tables, the flop model, and the
+C-ABI library's exported symbols."""
+import itertools
+import os
+import re
+
+import numpy as np
+
+from intception_amd import basis, codegen, flops_model, lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_contraction_normalised():
+    rng = np.random.default_rng(0)
+    for l in range(5):
+        e = 10 ** rng.uniform(-1, 2, size=4)
+        c = basis.normalize_contraction(e, rng.uniform(0.1, 1, size=4), l)
+        p = e[:, None] + e[None, :]
+        ov = basis._dfact(2 * l - 1) / (2 * p) ** l * (np.pi / p) ** 1.5
+        assert abs(c @ ov @ c - 1.0) < 1e-13
+
+
+def test_sph_orthonormal():
+    for l in range(7):
+        S = basis.SPH[l]
+        G = S.T @ basis._cart_overlap(l) @ S
+        assert np.abs(G - np.eye(2 * l + 1)).max() < 1e-13
+
+
+def test_sph_d_known_values():
+    S = basis.SPH[2]
+    # m = 0 column: z^2 - (x^2 + y^2) / 2
+    comps = basis.cart_components(2)
+    col = dict(zip(comps, S[:, 2]))
+    assert col[(0, 0, 2)] == 1.0 and col[(2, 0, 0)] == -0.5 and col[(0, 2, 0)] == -0.5
+
+
+def test_canonical_classes():
+    for cls in itertools.product(range(4), repeat=4):
+        can, perm = codegen.canonical(*cls)
+        assert sorted(perm) == [0, 1, 2, 3]
+        assert tuple(cls[p] for p in perm) == can
+        assert can[0] >= can[1] and can[2] >= can[3]
+        assert (can[0] + can[1], can[0]) <= (can[2] + can[3], can[2])
+        assert {perm[0], perm[1]} in ({0, 1}, {2, 3})
+    assert len(codegen.canonical_classes(2)) == 21
+
+
+def test_lane_chain_tables():
+    # every lane's chain of directions rebuilds its own component
+    for LX in range(7):
+        for e in codegen.comps_upto(LX):
+            t = [0, 0, 0]
+            steps = []
+            u = e
+            while sum(u) > 0:
+                i = codegen.first_nonzero(u)
+                steps.append((i, u[i] - 1))
+                u = codegen.dec(u, i)
+            for i, k in reversed(steps):
+                assert k == t[i]          # multiplier of the (t - 1_i) term
+                t[i] += 1
+            assert tuple(t) == e
+
+
+def test_generated_kernel_is_well_formed():
+    src, meta = codegen.emit_class_source((2, 1, 2, 2), True)
+    # every register variable is defined before use
+    defined = set()
+    for line in src.splitlines():
+        m = re.match(r"\s*const double (\w+) = (.*);", line)
+        if m:
+            for tok in re.findall(r"\b([vc]\d+_\d+|hk\d+_\d+|hb\d+_\d+)\b", m.group(2)):
+                assert tok in defined, tok
+            defined.add(m.group(1))
+    assert meta["G"] == 20 and meta["NCD"] == 25
+
+
+def test_flops_model_small_classes():
+    # (ps|ss): a-build of 3 components at m = 0, 2 terms each
+    assert flops_model.vrr_terms(1, 0, 0, 0) == 3 * 2
+    assert flops_model.vrr_terms(0, 0, 0, 0) == 0
+    assert flops_model.quartet_flops((0, 0, 0, 0), (1, 1, 1, 1), False) == 40 + 22 + 1 + 2
+
+
+def test_library_exports_every_declared_symbol():
+    if not os.path.exists(lib.LIB_PATH):
+        import pytest
+        pytest.skip("libintc.so not built")
+    hdr = open(os.path.join(ROOT, "include", "intc.h")).read()
+    hdr = re.sub(r"/\*.*?\*/", "", hdr, flags=re.S)
+    names = set(re.findall(r"\b(intc_\w+)\s*\(", hdr)) - {"intc_plan"}
+    L = lib.load()
+    for n in names:
+        assert hasattr(L, n), n
+    assert {s[0] for s in lib.SIGNATURES} == names
+    # host-only entry points work without a GPU
+    assert L.intc_eri_cont_cart_work_array_size(1, 1, 1, 1, 3, 3, 3, 3, 1, 1, 1, 1) == 25930
+    assert L.intc_set_boys_mode(7) != 0
+    assert b"boys mode" in L.intc_last_error()

@@ -1,0 +1,154 @@
+"""CPU: pin the oracle (C restatement of the reference's generated ERI code).
+
+* against an independent 40-digit McMurchie-Davidson evaluation (fixture
+  tests/golden/eri_mp_golden.npz, written by tests/golden/make_eri_golden.py)
+* closed form of (ss|ss)
+* invariants a dropped term or wrong index would break: 8-fold permutational
+  symmetry, translation invariance, Schwarz inequality, Cartesian vs spherical
+  consistency (sph = S^T cart S on every index)
+"""
+import itertools
+import math
+import os
+
+import numpy as np
+import pytest
+
+from conftest import assert_parity
+from intception_amd import basis
+from oracle import oracle
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden", "eri_mp_golden.npz")
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _exact_boys():
+    oracle.set_boys_mode(oracle.BOYS_EXACT)
+    yield
+    oracle.set_boys_mode(oracle.BOYS_REFERENCE)
+
+
+def _golden_cases():
+    g = np.load(GOLD)
+    nq = len(g["sph"])
+    for q in range(nq):
+        shells = []
+        for k in range(4):
+            s = 4 * q + k
+            n = int(g["nprim"][s])
+            shells.append({"l": int(g["l"][s]), "exps": g["exps"][s][:n], "coefs": g["coefs"][s][:n],
+                           "cen": g["cen"][s]})
+        yield shells, bool(g["sph"][q]), g["val"][g["off"][q]:g["off"][q + 1]], tuple(g["classes"][g["cls_id"][q]])
+
+
+@pytest.mark.parametrize("mode", [oracle.BOYS_EXACT, oracle.BOYS_REFERENCE])
+def test_oracle_matches_independent_md(mode):
+    oracle.set_boys_mode(mode)
+    n = 0
+    for shells, sph, ref, cls in _golden_cases():
+        got = oracle.eri_quartet(shells, sph, basis.SPH)
+        assert_parity(got, ref, what="oracle vs MD %s sph=%s" % (cls, sph))
+        n += 1
+    assert n >= 80
+
+
+def _shell(rng, l, k=2, box=3.0):
+    e = 10.0 ** rng.uniform(-1, 1.5, size=k)
+    return {"l": l, "exps": e, "coefs": basis.normalize_contraction(e, rng.uniform(0.1, 1, size=k), l),
+            "cen": rng.uniform(0, box, size=3)}
+
+
+def test_ssss_closed_form():
+    rng = np.random.default_rng(3)
+    for _ in range(20):
+        sh = [_shell(rng, 0, k=1) for _ in range(4)]
+        a, b, c, d = (s["exps"][0] for s in sh)
+        A, B, C, D = (s["cen"] for s in sh)
+        p, q = a + b, c + d
+        P = (a * A + b * B) / p
+        Q = (c * C + d * D) / q
+        T = p * q / (p + q) * np.sum((P - Q) ** 2)
+        F0 = 1.0 if T == 0 else 0.5 * math.sqrt(math.pi / T) * math.erf(math.sqrt(T))
+        val = (2 * math.pi ** 2.5 / (p * q * math.sqrt(p + q)) * math.exp(-a * b / p * np.sum((A - B) ** 2))
+               * math.exp(-c * d / q * np.sum((C - D) ** 2)) * F0)
+        val *= np.prod([s["coefs"][0] for s in sh])
+        got = oracle.eri_quartet(sh, False)
+        assert abs(got[0] - val) <= 1e-14 + 1e-13 * abs(val)
+
+
+def _cart_block(shells):
+    n = [basis.ncc(s["l"]) for s in shells]
+    return oracle.eri_quartet(shells, False).reshape(n)
+
+
+@pytest.mark.parametrize("cls", [(1, 0, 2, 1), (2, 1, 1, 1), (2, 2, 1, 0), (3, 1, 2, 0)])
+def test_permutational_symmetry(cls):
+    rng = np.random.default_rng(sum(cls))
+    sh = [_shell(rng, l) for l in cls]
+    ref = _cart_block(sh)
+    perms = [((1, 0, 2, 3), (1, 0, 2, 3)), ((0, 1, 3, 2), (0, 1, 3, 2)), ((2, 3, 0, 1), (2, 3, 0, 1)),
+             ((3, 2, 1, 0), (3, 2, 1, 0))]
+    for order, axes in perms:
+        got = _cart_block([sh[i] for i in order])
+        assert_parity(got.transpose(np.argsort(axes)).reshape(-1), ref.reshape(-1), rtol=1e-11,
+                      what="perm %s" % (order,))
+
+
+def test_translation_invariance():
+    rng = np.random.default_rng(11)
+    sh = [_shell(rng, l) for l in (2, 1, 1, 2)]
+    ref = _cart_block(sh).reshape(-1)
+    shift = np.array([1.7, -0.4, 2.2])
+    sh2 = [dict(s, cen=s["cen"] + shift) for s in sh]
+    assert_parity(_cart_block(sh2).reshape(-1), ref, rtol=1e-10, atol=1e-13, what="translation")
+
+
+def test_schwarz_inequality():
+    rng = np.random.default_rng(5)
+    for cls in [(1, 1, 2, 0), (2, 1, 2, 2), (0, 0, 2, 2)]:
+        sh = [_shell(rng, l, box=6.0) for l in cls]
+        abcd = _cart_block(sh)
+        abab = _cart_block([sh[0], sh[1], sh[0], sh[1]])
+        cdcd = _cart_block([sh[2], sh[3], sh[2], sh[3]])
+        da = np.einsum("ijij->ij", abab)
+        dc = np.einsum("ijij->ij", cdcd)
+        bound = np.sqrt(np.abs(da))[:, :, None, None] * np.sqrt(np.abs(dc))[None, None, :, :]
+        assert np.all(np.abs(abcd) <= bound * (1 + 1e-10) + 1e-15)
+
+
+def test_spherical_equals_transformed_cartesian():
+    rng = np.random.default_rng(9)
+    for cls in [(2, 1, 2, 0), (1, 2, 2, 2), (3, 0, 1, 2)]:
+        sh = [_shell(rng, l) for l in cls]
+        cart = _cart_block(sh)
+        S = [basis.SPH[l] for l in cls]
+        ref = np.einsum("abcd,ai,bj,ck,dl->ijkl", cart, *S).reshape(-1)
+        got = oracle.eri_quartet(sh, True, basis.SPH)
+        assert_parity(got, ref, what="sph vs S^T cart S %s" % (cls,))
+
+
+def test_general_contraction_and_strides():
+    # n_cont = 2 on two indices, custom strides: output layout of
+    # contracted_general.py:1808-1875
+    rng = np.random.default_rng(21)
+    sh = [_shell(rng, l, k=3) for l in (1, 0, 1, 1)]
+    c2 = rng.uniform(0.1, 1.0, size=(3, 2))
+    sh[0] = dict(sh[0], coefs=c2)
+    sh[3] = dict(sh[3], coefs=c2[:, ::-1].copy())
+    ns = [3, 1, 3, 3]
+    strides = [9, 100, 3, 1]
+    cstr = [1000, 0, 0, 2000]
+    out = oracle.eri_quartet(sh, False, strides=strides, ncont_strides=cstr, out_size=5000)
+    for ka, kd in itertools.product(range(2), range(2)):
+        single = [dict(sh[0], coefs=c2[:, ka]), sh[1], sh[2], dict(sh[3], coefs=c2[:, ::-1][:, kd])]
+        ref = oracle.eri_quartet(single, False).reshape(ns)
+        for a, c, d in itertools.product(range(3), range(3), range(3)):
+            assert out[a * 9 + ka * 1000 + c * 3 + d + kd * 2000] == pytest.approx(ref[a, 0, c, d], rel=1e-13,
+                                                                                   abs=1e-16)
+
+
+def test_work_array_size_formula():
+    # hand-evaluated from contracted_vrr_hrr_with_auxiliary.py:248-351 for (pp|pp), K = 3:
+    # VRR: 2 * (ncc0(2) * ncc0(1) * ncc0(1) * 81 + 4 + 1) = 2 * (10*4*4*81 + 5) = 25930
+    assert oracle.work_array_size(False, (1, 1, 1, 1), (3, 3, 3, 3)) == 25930
+    assert oracle.work_array_size(True, (0, 0, 0, 0), (1, 1, 1, 1)) == 4
