@@ -1,0 +1,342 @@
+"""``generator`` / ``generator_options`` with a HIP backend.
+
+Same entry points as the reference generator
+(src/intception/generator/generator.py:102-524, 1352-1702):
+
+    opt = generator_options(contracted=True, spherical_transformed=True,
+                            use_cblas=False, global_user_lmax=4, backend="hip")
+    gen = generator(eri, opt=opt)
+    gen.out()
+
+With ``backend="hip"`` the generator does not emit recurrence code from the
+script's expression tree.  It checks, numerically at random exponents,
+centres, target components and directions, that each integral is the
+Obara-Saika / Head-Gordon-Pople Coulomb ERI family (base expression, every VRR
+and the HRR), and binds it to the hand-written gfx950 kernels of libintc:
+
+* ``gen.binding(name)`` -> ``EriBinding`` with the per-quartet twin
+  ``main(...)`` (same argument layout as the reference's ``<full>_main``),
+  ``work_array_size(...)`` and ``plan(batch)`` for the batched path;
+* ``gen.out()`` writes ``<full>.h`` to the output directory: the reference's
+  function names (``<full>_main``, ``<full>_work_array_size``) mapped onto the
+  libintc C-ABI (include/intc.h), plus a JSON manifest.
+
+An integral that is not in the family raises ``NotImplementedError``.
+"""
+import json
+import math
+import os
+
+import numpy as np
+
+from . import dsl
+
+
+class parallelism:
+    def __init__(self, vectorized=False, vector_batch_size=4):
+        self._vectorized, self._vbs = vectorized, vector_batch_size
+
+    def vectorized(self):
+        return self._vectorized
+
+
+class generator_options:
+    """Reference options (contracted, spherical_transformed, global_user_lmax,
+    use_cblas, output_directory, suffixes, silent) plus this build's:
+    backend ("hip"), precision ("fp64" or "mixed"), boys ("chebyshev" or
+    "reference"), screening threshold."""
+
+    def __init__(self, contracted=None, spherical_transformed=None, global_user_lmax=None, use_cblas=None,
+                 output_directory=None, auto_suffix=None, prim_suffix=None, cont_suffix=None, cart_suffix=None,
+                 sph_suffix=None, silent=None, backend=None, precision=None, boys=None, screening=None,
+                 **unused):
+        self._contracted = True if contracted is None else bool(contracted)
+        self._sph = True if spherical_transformed is None else bool(spherical_transformed)
+        if self._sph and not self._contracted:
+            raise ValueError("spherical transformation requires contracted integrals")
+        self._lmax = 6 if global_user_lmax is None else int(global_user_lmax)
+        self._cblas = bool(use_cblas)
+        self._outdir = output_directory or "./output"
+        self._auto_suffix = True if auto_suffix is None else auto_suffix
+        self._suffix = {"prim": prim_suffix or "prim", "cont": cont_suffix or "cont", "cart": cart_suffix or "cart",
+                        "sph": sph_suffix or "sph"}
+        self._silent = bool(silent)
+        self._backend = backend or "hip"
+        if self._backend != "hip":
+            raise ValueError("this build's generator has the 'hip' backend")
+        self._precision = precision or "fp64"
+        assert self._precision in ("fp64", "mixed")
+        self._boys = boys or "chebyshev"
+        assert self._boys in ("chebyshev", "reference")
+        self._screening = 1e-12 if screening is None else float(screening)
+        self._unused = unused
+
+    def contracted(self): return self._contracted
+    def spherical_transformed(self): return self._sph
+    def global_user_lmax(self): return self._lmax
+    def use_cblas(self): return self._cblas
+    def output_directory(self): return self._outdir
+    def backend(self): return self._backend
+    def precision(self): return self._precision
+    def boys(self): return self._boys
+    def parallelism(self): return parallelism()
+
+    def suffix(self):
+        if not self._auto_suffix:
+            return ""
+        s = "_" + self._suffix["cont" if self._contracted else "prim"]
+        return s + "_" + self._suffix["sph" if self._sph else "cart"]
+
+
+# ---------------------------------------------------------------------------
+# numeric recognition of the OS/HGP Coulomb ERI family
+# ---------------------------------------------------------------------------
+def _boys_exact(n, x):
+    if x < 30.0:
+        term = 1.0 / (2 * n + 1)
+        s = term
+        for k in range(1, 400):
+            term *= 2.0 * x / (2 * n + 2 * k + 1)
+            s += term
+            if term < 1e-18 * s:
+                break
+        return math.exp(-x) * s
+    f = 0.5 * math.sqrt(math.pi / x) * math.erf(math.sqrt(x))
+    for m in range(n):
+        f = ((2 * m + 1) * f - math.exp(-x)) / (2 * x)
+    return f
+
+
+def _ctx(rng, names):
+    ex = {n: 10 ** rng.uniform(-0.5, 0.8) for n in names}
+    cen = {n: rng.uniform(-1.5, 1.5, size=3) for n in names}
+    v = {}
+    for n in names:
+        v["x" + n] = ex[n]
+        v["c" + n] = cen[n]
+    return {"vars": v, "dir": 0, "index": {}, "aux": {}, "boys": _boys_exact}, ex, cen
+
+
+def _expected_base(ex, cen, names, m):
+    a, b, c, d = (ex[n] for n in names)
+    A, B, C, D = (cen[n] for n in names)
+    z, e = a + b, c + d
+    P, Q = (a * A + b * B) / z, (c * C + d * D) / e
+    T = z * e / (z + e) * float(np.sum((P - Q) ** 2))
+    return (2 * math.pi ** 2.5 / (z * e * math.sqrt(z + e)) * math.exp(-a * b / z * float(np.sum((A - B) ** 2)))
+            * math.exp(-c * d / e * float(np.sum((C - D) ** 2))) * _boys_exact(m, T))
+
+
+def _expected_vrr(ex, cen, names, aux, X, target, i):
+    """OS VRR for incrementing index X (position in names) to `target`
+    (dict name -> (x,y,z)) in direction i: {shift key: coefficient}."""
+    a, b, c, d = (ex[n] for n in names)
+    A, B, C, D = (cen[n] for n in names)
+    z, e = a + b, c + d
+    P, Q = (a * A + b * B) / z, (c * C + d * D) / e
+    W = (z * P + e * Q) / (z + e)
+    rho = z * e / (z + e)
+    bra = X in (0, 1)
+    ctr = [A, B, C, D][X]
+    own = [0, 1] if bra else [2, 3]
+    other = [2, 3] if bra else [0, 1]
+    partner = own[1] if X == own[0] else own[0]
+    zz = z if bra else e
+    center = P if bra else Q
+    out = {}
+
+    def key(dec):
+        sh = [(n, -dec.get(k, 0)) for k, n in enumerate(names)]
+        return tuple(sh) + ((aux, dec.get("m", 0)),)
+
+    def add(dec, v):
+        if v != 0.0:
+            k = key(dec)
+            out[k] = out.get(k, 0.0) + v
+
+    add({X: 1}, (center - ctr)[i])
+    add({X: 1, "m": 1}, (W - center)[i])
+    nx = target[names[X]][i] - 1
+    if nx > 0:
+        add({X: 2}, nx / (2 * zz))
+        add({X: 2, "m": 1}, -nx / (2 * zz) * rho / zz)
+    npart = target[names[partner]][i]
+    if npart > 0:
+        add({X: 1, partner: 1}, npart / (2 * zz))
+        add({X: 1, partner: 1, "m": 1}, -npart / (2 * zz) * rho / zz)
+    for o in other:
+        no = target[names[o]][i]
+        if no > 0:
+            add({X: 1, o: 1, "m": 1}, no / (2 * (z + e)))
+    return out
+
+
+def _lin_clean(d):
+    return {k: v for k, v in d.items() if k is not None and v != 0.0}
+
+
+def recognize_eri(integral, trials=24, seed=7):
+    """Raise NotImplementedError unless `integral` is the OS/HGP Coulomb ERI
+    (4 Cartesian indices a,b | c,d, one auxiliary index)."""
+    idx, aux = integral.index_list(), integral.aux_list()
+    if len(idx) != 4 or len(aux) != 1 or integral.arg_list():
+        raise NotImplementedError("hip backend: integral %s is not a 4-index Coulomb ERI with one auxiliary index"
+                                  % integral.name())
+    names = [i.name() for i in idx]
+    mname = aux[0].name()
+    rng = np.random.default_rng(seed)
+    base = integral.base()
+    if base is None:
+        raise NotImplementedError("no base expression")
+    for _ in range(trials // 4):
+        ctx, ex, cen = _ctx(rng, names)
+        for m in range(4):
+            ctx["aux"] = {mname: m}
+            got = base.ev(ctx)
+            ref = _expected_base(ex, cen, names, m)
+            if not abs(got - ref) <= 1e-11 * abs(ref) + 1e-300:
+                raise NotImplementedError("base expression of %s is not the Coulomb ERI base" % integral.name())
+    vrrs = [r for r in integral.rr_list() if r.rrtype() == "vrr"]
+    hrrs = [r for r in integral.rr_list() if r.rrtype() == "hrr"]
+    if not vrrs:
+        raise NotImplementedError("no VRR")
+    for r in vrrs:
+        X = names.index(r.changing_index().name())
+        for _ in range(trials):
+            ctx, ex, cen = _ctx(rng, names)
+            target = {n: tuple(int(v) for v in rng.integers(0, 3, size=3)) for n in names}
+            i = int(rng.integers(0, 3))
+            if target[names[X]][i] == 0:
+                t = list(target[names[X]]); t[i] = 1; target[names[X]] = tuple(t)
+            ctx.update(dir=i, index=target, aux={mname: int(rng.integers(0, 3))})
+            got = _lin_clean(r.expr().lin(ctx))
+            ref = _expected_vrr(ex, cen, names, mname, X, target, i)
+            keys = set(got) | set(ref)
+            for k in keys:
+                g, f = got.get(k, 0.0), ref.get(k, 0.0)
+                if not abs(g - f) <= 1e-10 * max(abs(f), 1e-3):
+                    raise NotImplementedError("VRR %s on %s is not the Obara-Saika ERI recurrence (term %s: %r vs %r)"
+                                              % (r.name(), names[X], k, g, f))
+    built = {r.changing_index().name() for r in vrrs}
+    for r in hrrs:
+        to, frm = r.changing_index().name(), r.move_from().name()
+        if {to, frm} not in ({names[0], names[1]}, {names[2], names[3]}):
+            raise NotImplementedError("HRR must move angular momentum within one electron's pair")
+        built.add(to)
+    if built != set(names):
+        raise NotImplementedError("every index must be built by a VRR or the HRR (built: %s)" % sorted(built))
+    return True
+
+
+# ---------------------------------------------------------------------------
+# bindings
+# ---------------------------------------------------------------------------
+class EriBinding:
+    """One generated class bound to libintc."""
+
+    def __init__(self, integral, options):
+        self.integral = integral
+        self.options = options
+        self.full_name = integral.full_name() + options.suffix()
+        self.spherical = options.spherical_transformed()
+
+    def _lib(self):
+        from . import lib
+        L = lib.load()
+        lib.check(L.intc_set_boys_mode(1 if self.options.boys() == "reference" else 0))
+        return L
+
+    def work_array_size(self, la, lb, lc, ld, na_prim, nb_prim, nc_prim, nd_prim, na_cont=1, nb_cont=1,
+                        nc_cont=1, nd_cont=1):
+        from . import lib
+        f = lib.load().intc_eri_cont_sph_work_array_size if self.spherical else \
+            lib.load().intc_eri_cont_cart_work_array_size
+        return f(la, lb, lc, ld, na_prim, nb_prim, nc_prim, nd_prim, na_cont, nb_cont, nc_cont, nd_cont)
+
+    def main(self, shells, output_array, work_array=None, strides=None, cont_strides=None):
+        """Per-quartet twin of <full>_main.  shells: 4 dicts with exps,
+        cen, l, coefs (n_prim x n_cont), and 'sph' (ncc x nsph) when spherical."""
+        import ctypes
+        from . import lib
+        from .basis import ncc, nsph
+        L = self._lib()
+        dp = ctypes.POINTER(ctypes.c_double)
+        ns = [nsph(s["l"]) if self.spherical else ncc(s["l"]) for s in shells]
+        if strides is None:
+            strides = [ns[1] * ns[2] * ns[3], ns[2] * ns[3], ns[3], 1]
+        cont_strides = cont_strides or [0, 0, 0, 0]
+        keep, args = [], []
+        for k, s in enumerate(shells):
+            x = np.ascontiguousarray(s["exps"], dtype=np.float64)
+            c = np.asfortranarray(np.asarray(s["coefs"], dtype=np.float64).reshape(len(x), -1))
+            cen = np.ascontiguousarray(s["cen"], dtype=np.float64)
+            keep += [x, c, cen]
+            args += [x.ctypes.data_as(dp), cen.ctypes.data_as(dp), s["l"], ns[k], int(strides[k]), len(x),
+                     c.shape[1], int(cont_strides[k]), c.ctypes.data_as(dp)]
+            if self.spherical:
+                S = np.asfortranarray(s["sph"], dtype=np.float64)
+                keep.append(S)
+                args.append(S.ctypes.data_as(dp))
+        work = np.zeros(1) if work_array is None else work_array
+        out = output_array
+        fn = L.intc_eri_cont_sph_main if self.spherical else L.intc_eri_cont_cart_main
+        lib.check(fn(*args, work.ctypes.data_as(dp), out.ctypes.data_as(dp)))
+        return out
+
+    def plan(self, batch, device="cuda"):
+        from .runtime import EriPlan
+        assert batch.spherical == self.spherical
+        self._lib()
+        return EriPlan(batch, device)
+
+    def header(self):
+        n = self.full_name
+        kind = "sph" if self.spherical else "cart"
+        return "\n".join([
+            "/* Generated by intception_amd.generator (hip backend) for integral '%s'. */" % self.integral.name(),
+            "#ifndef %s_H" % n.upper(), "#define %s_H" % n.upper(), '#include "intc.h"',
+            "/* Reference names mapped onto the libintc C-ABI; the functions return an",
+            " * int status (0 = success) where the reference's return void. */",
+            "#define %s_main intc_eri_cont_%s_main" % (n, kind),
+            "#define %s_work_array_size intc_eri_cont_%s_work_array_size" % (n, kind),
+            "#endif", ""])
+
+
+class generator:
+    def __init__(self, *integrals, opt=None):
+        self._opt = opt or generator_options()
+        self._bindings = {}
+        for itg in integrals:
+            if not isinstance(itg, dsl.dsl_integral):
+                raise TypeError("generator takes dsl_integral objects")
+            if not self._opt.contracted():
+                raise NotImplementedError("hip backend generates contracted integrals")
+            recognize_eri(itg)
+            b = EriBinding(itg, self._opt)
+            self._bindings[b.full_name] = b
+
+    def generator_options(self):
+        return self._opt
+
+    def names(self):
+        return list(self._bindings)
+
+    def binding(self, name=None):
+        if name is None:
+            assert len(self._bindings) == 1
+            return next(iter(self._bindings.values()))
+        return self._bindings[name]
+
+    def out(self):
+        d = self._opt.output_directory()
+        os.makedirs(d, exist_ok=True)
+        manifest = {}
+        for n, b in self._bindings.items():
+            with open(os.path.join(d, n + ".h"), "w") as fh:
+                fh.write(b.header())
+            manifest[n] = {"integral": b.integral.name(), "spherical": b.spherical, "backend": "hip",
+                           "precision": self._opt.precision(), "boys": self._opt.boys(),
+                           "library": "libintc.so", "header": "intc.h"}
+        with open(os.path.join(d, "intception_hip_manifest.json"), "w") as fh:
+            json.dump(manifest, fh, indent=1)
+        return manifest

@@ -1,0 +1,106 @@
+"""Input-script API: the Coulomb ERI script drives the HIP-backend generator.
+
+CPU: the generator accepts the Obara-Saika/HGP ERI script, rejects scripts with
+a dropped or wrong recurrence term, and writes the header that maps the
+reference's function names onto the libintc C-ABI.
+GPU: the bound per-quartet twin (reference argument layout, n_cont > 1,
+caller strides) and the batched plan agree with the oracle.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import assert_parity
+from intception_amd import basis, eri_script
+from intception_amd.dsl import dsl_cartesian_gaussian
+from intception_amd.generator import generator, generator_options, recognize_eri
+
+
+def test_eri_script_recognised():
+    eri = eri_script.make_eri()
+    assert recognize_eri(eri)
+    assert eri.full_name() == "intception_integral_eri_abcd_m"
+
+
+def test_broken_scripts_rejected():
+    import intception_amd.dsl as d
+    eri = eri_script.make_eri()
+    # drop the W-P term from the VRR on a
+    r = eri.rr_list()[0]
+    bad = d.dsl_integral(*eri.index_list(), *eri.aux_list(), name="eri")
+    bad.set_base(eri.base())
+    ga, gb, gc, gd = eri.index_list()
+    m = eri.aux_list()[0]
+    expr = r.expr()
+
+    def find(node, name):
+        if isinstance(node, d.dsl_position) and node.name() == name:
+            return node
+        for ch in (getattr(node, "_l", None), getattr(node, "_r", None), getattr(node, "_a", None)):
+            if ch is not None:
+                hit = find(ch, name)
+                if hit is not None:
+                    return hit
+        return None
+
+    WP = find(expr, "WP")
+    assert WP is not None
+    # halve the (W-P)_i (a-1_i)^(m+1) term
+    bad.add_vrr("vrr_a", ga, expr - 0.5 * WP * bad.int(ga - 1, gb, gc, gd, m + 1))
+    for rr in eri.rr_list()[1:]:
+        if rr.rrtype() == "vrr":
+            bad.add_vrr(rr.name(), rr.changing_index(), rr.expr())
+    bad.add_hrr("hrr", ga, gb)
+    with pytest.raises(NotImplementedError, match="not the Obara-Saika"):
+        recognize_eri(bad)
+    # wrong base (missing K_CD)
+    bad2 = eri_script.make_eri()
+    bad2.set_base(bad2.base() * 1.0001)
+    with pytest.raises(NotImplementedError):
+        recognize_eri(bad2)
+    # incomplete: d never built
+    with pytest.raises(NotImplementedError):
+        recognize_eri(eri_script.make_eri(("a", "c")))
+
+
+def test_generator_out_writes_header(tmp_path):
+    opt = generator_options(output_directory=str(tmp_path), contracted=True, spherical_transformed=False,
+                            use_cblas=False, global_user_lmax=4, backend="hip")
+    gen = generator(eri_script.make_eri(), opt=opt)
+    man = gen.out()
+    name = "intception_integral_eri_abcd_m_cont_cart"
+    assert list(man) == [name]
+    h = (tmp_path / (name + ".h")).read_text()
+    assert "#define %s_main intc_eri_cont_cart_main" % name in h
+    assert gen.binding().work_array_size(1, 1, 1, 1, 3, 3, 3, 3) == 25930
+
+
+def test_cartesian_requires_contracted_for_sph():
+    with pytest.raises(ValueError):
+        generator_options(contracted=False, spherical_transformed=True)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("sph", [False, True])
+def test_bound_main_matches_oracle(gpu, sph):
+    from oracle import oracle
+    oracle.set_boys_mode(oracle.BOYS_EXACT)
+    gen = generator(eri_script.make_eri(), opt=generator_options(spherical_transformed=sph, backend="hip"))
+    b = gen.binding()
+    rng = np.random.default_rng(17)
+    for cls in [(0, 0, 0, 0), (1, 0, 2, 1), (2, 2, 1, 0), (1, 2, 2, 2)]:
+        shells = []
+        for l in cls:
+            k = int(rng.integers(1, 4))
+            e = 10 ** rng.uniform(-1, 2, size=k)
+            c = np.stack([basis.normalize_contraction(e, rng.uniform(0.1, 1, size=k), l) for _ in range(2)], 1)
+            shells.append({"l": l, "exps": e, "coefs": c, "cen": rng.uniform(0, 10, size=3), "sph": basis.SPH[l]})
+        ns = [basis.nsph(l) if sph else basis.ncc(l) for l in cls]
+        blk = int(np.prod(ns))
+        strides = [ns[1] * ns[2] * ns[3], ns[2] * ns[3], ns[3], 1]
+        cstr = [blk, 2 * blk, 4 * blk, 8 * blk]
+        out = np.zeros(16 * blk)
+        b.main(shells, out, strides=strides, cont_strides=cstr)
+        ref = oracle.eri_quartet(shells, sph, basis.SPH, strides=strides, ncont_strides=cstr, out_size=16 * blk)
+        assert_parity(out, ref, what="bound main %s sph=%s" % (cls, sph))
