@@ -83,21 +83,33 @@ int intc_plan_create(intc_plan **plan, int spherical, int n_shell, const int *sh
                      const int *shell_off, const double *shell_cen, long long n_quartet, const int *quartets,
                      const long long *out_off);
 int intc_plan_num_bins(const intc_plan *plan);
-int intc_plan_bin_info(const intc_plan *plan, int bin, int *cls4, int *nq, int *team_groups, int *grid, int *G);
-int intc_plan_set_bin_tuning(intc_plan *plan, int bin, int team_groups, int grid);
+/* prec: 0 = FP64 kernel, 1 = kernel with FP32 recurrences (FP64 Boys,
+ * prefactors, contraction sums and contracted HRR/transform). */
+int intc_plan_bin_info(const intc_plan *plan, int bin, int prec, int *cls4, int *nq, int *team_groups, int *grid,
+                       int *G);
+int intc_plan_set_bin_tuning(intc_plan *plan, int bin, int prec, int team_groups, int grid);
 /* Device pointers.  sph: S_l for l = 0..7 concatenated, each ncc(l) x (2l+1)
  * column-major (NULL for Cartesian plans).  stream: hipStream_t or NULL. */
 int intc_plan_build_pairs(intc_plan *plan, const double *exps, const double *coefs, const double *shell_cen,
                           void *stream);
-int intc_plan_run_bin(intc_plan *plan, int bin, const double *sph, double *out, void *stream);
+int intc_plan_run_bin(intc_plan *plan, int bin, int prec, const double *sph, double *out, void *stream);
+/* All quartets in FP64. */
 int intc_plan_execute(intc_plan *plan, const double *exps, const double *coefs, const double *shell_cen,
                       const double *sph, double *out, void *stream);
-/* One hipGraph holding the pair builder and every class-bin kernel; with
- * n_streams > 1 the bins are independent nodes that run concurrently.  The
- * graph binds the given device pointers; replay with intc_plan_graph_launch. */
+/* Per-quartet route (device int8 per caller quartet: 0 FP64, 1 FP32
+ * recurrences, 2 skip = block set to zero), compacted on the device into
+ * per-class FP64/FP32 queues, then both kernels per class. */
+int intc_plan_execute_routed(intc_plan *plan, const double *exps, const double *coefs, const double *shell_cen,
+                             const double *sph, const signed char *route, double *out, void *stream);
+/* One hipGraph holding the pair builder, the route compaction (route != NULL)
+ * and every class-bin kernel; with n_streams > 1 the bins are independent
+ * nodes that run concurrently.  The graph binds the given device pointers
+ * (route is read at replay time); replay with intc_plan_graph_launch. */
 int intc_plan_graph_create(intc_plan *plan, const double *exps, const double *coefs, const double *shell_cen,
-                           const double *sph, double *out, int n_streams, void *stream);
+                           const double *sph, const signed char *route, double *out, int n_streams, void *stream);
 int intc_plan_graph_launch(intc_plan *plan, void *stream);
+/* FP64/FP32 queue lengths per bin of the last routed execution (2 ints per bin). */
+int intc_plan_route_counts(intc_plan *plan, int *counts);
 int intc_plan_destroy(intc_plan *plan);
 
 #ifdef __cplusplus

@@ -37,9 +37,9 @@ def grid_for(nq, G, T, mode):
     return max(1, min(full, N_CU * int(mode)))
 
 
-def _key(b, spherical, boys_mode, mean_p):
-    return "%d%d%d%d_%s_b%d_p%d" % (*b["cls"], "s" if spherical else "c", boys_mode,
-                                   int(round(math.log2(max(mean_p, 1.0)))))
+def _key(b, spherical, boys_mode, mean_p, prec=0):
+    return "%s%d%d%d%d_%s_b%d_p%d" % ("d" if prec == 0 else "f", *b["cls"], "s" if spherical else "c", boys_mode,
+                                     int(round(math.log2(max(mean_p, 1.0)))))
 
 
 def load_cache(path=CACHE):
@@ -67,17 +67,18 @@ def bin_mean_prims(plan):
     return out
 
 
-def apply_cache(plan, boys_mode=0, cache=None):
+def apply_cache(plan, boys_mode=0, cache=None, precs=(0, 1)):
     cache = load_cache() if cache is None else cache
     means = bin_mean_prims(plan)
     n = 0
-    for r in plan.bins():
-        k = _key(r, plan.batch.spherical, boys_mode, means[r["bin"]])
-        if k in cache:
-            T, mode = cache[k]["T"], cache[k]["grid"]
-            if G_ok(r["G"], T):
-                plan.set_tuning(r["bin"], T, grid_for(r["nq"], r["G"], T, mode))
-                n += 1
+    for prec in precs:
+        for r in plan.bins(prec):
+            k = _key(r, plan.batch.spherical, boys_mode, means[r["bin"]], prec)
+            if k in cache:
+                T, mode = cache[k]["T"], cache[k]["grid"]
+                if G_ok(r["G"], T):
+                    plan.set_tuning(r["bin"], T, grid_for(r["nq"], r["G"], T, mode), prec)
+                    n += 1
     return n
 
 
@@ -85,36 +86,39 @@ def G_ok(G, T):
     return 1 <= T and G * T <= 64
 
 
-def tune(plan, out, reps=3, boys_mode=0, grid_modes=("auto", 4, 8), save=True, log=None, save_path=None):
-    """Time every candidate per bin and set the fastest; returns the table."""
+def tune(plan, out, reps=3, boys_mode=0, grid_modes=("auto", 4, 8), save=True, log=None, save_path=None,
+         precs=(0,)):
+    """Time every candidate per bin (full bins) and set the fastest; returns the table."""
     import torch
     stream = torch.cuda.current_stream(plan.device)
     ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     means = bin_mean_prims(plan)
     cache = load_cache()
     table = []
-    for r in plan.bins():
-        best = None
-        for T in team_candidates(r["G"]):
-            for mode in grid_modes:
-                grid = grid_for(r["nq"], r["G"], T, mode)
-                plan.set_tuning(r["bin"], T, grid)
-                plan.run_bin(r["bin"], out)  # warm
-                ev0.record(stream)
-                for _ in range(reps):
-                    plan.run_bin(r["bin"], out)
-                ev1.record(stream)
-                torch.cuda.synchronize(plan.device)
-                ms = ev0.elapsed_time(ev1) / reps
-                if best is None or ms < best[0]:
-                    best = (ms, T, mode, grid)
-        ms, T, mode, grid = best
-        plan.set_tuning(r["bin"], T, grid)
-        cache[_key(r, plan.batch.spherical, boys_mode, means[r["bin"]])] = {"T": T, "grid": mode,
-                                                                            "ms": round(ms, 4)}
-        table.append({"cls": r["cls"], "nq": r["nq"], "T": T, "grid": mode, "ms": ms})
-        if log:
-            log("tune %s nq=%d -> T=%d grid=%s %.3f ms" % (r["cls"], r["nq"], T, mode, ms))
+    for prec in precs:
+        for r in plan.bins(prec):
+            best = None
+            for T in team_candidates(r["G"]):
+                for mode in grid_modes:
+                    grid = grid_for(r["nq"], r["G"], T, mode)
+                    plan.set_tuning(r["bin"], T, grid, prec)
+                    plan.run_bin(r["bin"], out, prec)  # warm
+                    ev0.record(stream)
+                    for _ in range(reps):
+                        plan.run_bin(r["bin"], out, prec)
+                    ev1.record(stream)
+                    torch.cuda.synchronize(plan.device)
+                    ms = ev0.elapsed_time(ev1) / reps
+                    if best is None or ms < best[0]:
+                        best = (ms, T, mode, grid)
+            ms, T, mode, grid = best
+            plan.set_tuning(r["bin"], T, grid, prec)
+            cache[_key(r, plan.batch.spherical, boys_mode, means[r["bin"]], prec)] = {"T": T, "grid": mode,
+                                                                                      "ms": round(ms, 4)}
+            table.append({"cls": r["cls"], "nq": r["nq"], "prec": prec, "T": T, "grid": mode, "ms": ms})
+            if log:
+                log("tune %s%s nq=%d -> T=%d grid=%s %.3f ms" % ("d" if prec == 0 else "f", r["cls"], r["nq"], T,
+                                                                mode, ms))
     if save:
         try:
             save_cache(cache)

@@ -216,6 +216,54 @@ def make_random_quartets(seed, classes, nprim_choices, spherical, box=10.0, log_
                         meta={"seed": seed})
 
 
+def batch_from_shells(quartet_shells, spherical):
+    """QuartetBatch from a list of quartets, each a list of 4 shell dicts
+    (l, exps, coefs, cen); every shell gets its own table entry."""
+    ls, nps, cen, ex, co = [], [], [], [], []
+    for sh in quartet_shells:
+        for s in sh:
+            ls.append(int(s["l"]))
+            nps.append(len(s["exps"]))
+            cen.append(np.asarray(s["cen"], dtype=np.float64))
+            ex.append(np.asarray(s["exps"], dtype=np.float64))
+            co.append(np.asarray(s["coefs"], dtype=np.float64).reshape(-1))
+    nps = np.array(nps, dtype=np.int32)
+    off = np.zeros(len(nps), dtype=np.int32)
+    off[1:] = np.cumsum(nps)[:-1]
+    q = np.arange(len(ls), dtype=np.int32).reshape(-1, 4)
+    return QuartetBatch(bool(spherical), np.array(ls, dtype=np.int32), nps, off, np.array(cen),
+                        np.concatenate(ex), np.concatenate(co), q)
+
+
+def unit_s_shell():
+    """The unit s-shell of three-centre integrals (ab|P) = (ab|P s): exponent 0,
+    coefficient 1, so the ket pair has eta = exponent of P, Q = C, K_CD = 1."""
+    return 0, 1, np.array([0.0]), np.array([1.0])
+
+
+def random_three_center(seed, n, lmax_orb=2, lmax_aux=4, spherical=True, box=10.0):
+    """n quartets (a b | P s): orbital shells l <= lmax_orb, K in 1..6,
+    exponents 10^U[-1,2); auxiliary shells l <= lmax_aux, K = 1, exponents
+    10^U[log .05, log 200); one shared unit s-shell."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    lo = rng.integers(0, lmax_orb + 1, size=2 * n)
+    ko = rng.integers(1, 7, size=2 * n)
+    so = random_shells(rng, lo, ko, box, (-1.0, 2.0))
+    la = rng.integers(0, lmax_aux + 1, size=n)
+    sa = random_shells(rng, la, np.ones(n, dtype=np.int32), box, (np.log10(0.05), np.log10(200.0)))
+    ls = np.concatenate([so[0], sa[0], [0]]).astype(np.int32)
+    nprims = np.concatenate([so[1], sa[1], [1]]).astype(np.int32)
+    cen = np.concatenate([so[3], sa[3], np.zeros((1, 3))])
+    exps = np.concatenate([so[4], sa[4], [0.0]])
+    coefs = np.concatenate([so[5], sa[5], [1.0]])
+    off = np.zeros(len(ls), dtype=np.int32)
+    off[1:] = np.cumsum(nprims)[:-1]
+    unit = len(ls) - 1
+    q = np.stack([np.arange(0, 2 * n, 2), np.arange(1, 2 * n, 2), 2 * n + np.arange(n),
+                  np.full(n, unit)], axis=1).astype(np.int32)
+    return QuartetBatch(bool(spherical), ls, nprims, off, cen, exps, coefs, q, meta={"seed": seed})
+
+
 CONFIG0_CLASSES = [(0, 0, 0, 0), (1, 0, 0, 0), (1, 0, 1, 0), (1, 1, 0, 0), (1, 1, 1, 1)]
 CONFIG2_CLASSES = [(2, 2, 2, 2), (3, 2, 3, 2), (3, 3, 2, 2), (3, 3, 3, 3)]
 

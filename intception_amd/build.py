@@ -25,8 +25,9 @@ FLAGS = ["-O3", "-fPIC", "-std=c++17", "--offload-arch=" + ARCH,
          "-Wno-unused-command-line-argument"]
 
 # classes compiled into the library: every canonical class with shells up to
-# l = 2 (Cartesian and spherical), plus the three-centre and high-l sets
-EXTRA_CLASSES = []
+# l = 2 (Cartesian and spherical), plus the three-centre set (ab|P s) with
+# a, b <= d and auxiliary P up to g (the unit s-shell has exponent 0)
+EXTRA_CLASSES = [(la, lb, lp, 0) for la in range(3) for lb in range(3) for lp in range(3, 5)]
 
 
 def _digest(paths, extra=""):
@@ -76,7 +77,7 @@ def build(lmax=2, jobs=None, verbose=True):
         raise RuntimeError("link failed:\n" + r.stderr[-4000:])
     os.replace(LIB + ".tmp", LIB)
     if verbose:
-        print("libintc.so: %d classes x 2 outputs, %d objects rebuilt" % (len(classes), rebuilt))
+        print("libintc.so: %d classes x 2 outputs x 2 precisions, %d objects rebuilt" % (len(classes), rebuilt))
     return LIB
 
 

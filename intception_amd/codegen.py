@@ -104,32 +104,39 @@ def kernel_name(cls, spherical, prec="d"):
     return "intc_eri_%s%d%d%d%d_%s" % (prec, *cls, "s" if spherical else "c")
 
 
-def _chain_lines(LX, M):
-    """Bra chain: c{s}_{m} for s = 0..LX, m = 0..M-s; c0 given."""
+def _chain_lines(LX, M, R="double"):
+    """Bra chain: c{s}_{m} for s = 0..LX, m = 0..M-s; c0 given (type R)."""
+    one = "1.0f" if R == "float" else "1.0"
+    zero = "0.0f" if R == "float" else "0.0"
     L = []
     for s in range(1, LX + 1):
         L.append("const int d%d = (cdir >> %d) & 3;" % (s, 2 * (s - 1)))
-        L.append("const double pa%d = d%d == 0 ? PAx : (d%d == 1 ? PAy : (d%d == 2 ? PAz : 1.0));" % ((s,) * 4))
-        L.append("const double wp%d = d%d == 0 ? WPx : (d%d == 1 ? WPy : (d%d == 2 ? WPz : 0.0));" % ((s,) * 4))
+        L.append("const %s pa%d = d%d == 0 ? PAx : (d%d == 1 ? PAy : (d%d == 2 ? PAz : %s));"
+                 % (R, s, s, s, s, one))
+        L.append("const %s wp%d = d%d == 0 ? WPx : (d%d == 1 ? WPy : (d%d == 2 ? WPz : %s));"
+                 % (R, s, s, s, s, zero))
         if s >= 2:
-            L.append("const double kk%d = (double)((ckk >> %d) & 15) * o2z;" % (s, 4 * (s - 1)))
+            L.append("const %s kk%d = (%s)((ckk >> %d) & 15) * o2z;" % (R, s, R, 4 * (s - 1)))
         for m in range(M - s + 1):
             expr = "pa%d * c%d_%d + wp%d * c%d_%d" % (s, s - 1, m, s, s - 1, m + 1)
             if s >= 2:
                 expr += " + kk%d * (c%d_%d - roz * c%d_%d)" % (s, s - 2, m, s - 2, m + 1)
-            L.append("const double c%d_%d = %s;" % (s, m, expr))
+            L.append("const %s c%d_%d = %s;" % (R, s, m, expr))
     return L
 
 
-def _ket_lines(LX, LR, foff, prec="d"):
-    """Ket OS VRR in registers: V{f}_{m} for |f| <= LR, m <= LR-|f|.
+def _ket_lines(LX, LR, R="double"):
+    """Ket OS VRR in registers: V{f}_{m} for |f| <= LR, m <= LR-|f| (type R).
 
     Returns (lines, name(f, m))."""
     L = []
+    shfl = "intc_shflf" if R == "float" else "intc_shfl"
+    lit = "f" if R == "float" else ""
+
     def V(f, m):
         return "v%d_%d" % (cidx(f), m)
     for m in range(LR + 1):
-        L.append("const double %s = c%d_%d;" % (V((0, 0, 0), m), LX, m))
+        L.append("const %s %s = c%d_%d;" % (R, V((0, 0, 0), m), LX, m))
     for n in range(1, LR + 1):
         for t in cart_components(n):
             i = first_nonzero(t)
@@ -140,10 +147,10 @@ def _ket_lines(LX, LR, foff, prec="d"):
                 expr = "QC%s * %s + WQ%s * %s" % (ax, V(f1, m), ax, V(f1, m + 1))
                 if k1 > 0:
                     f2 = dec(f1, i)
-                    expr += " + %d.0 * o2e * (%s - roe * %s)" % (k1, V(f2, m), V(f2, m + 1))
+                    expr += " + %d.0%s * o2e * (%s - roe * %s)" % (k1, lit, V(f2, m), V(f2, m + 1))
                 if LX > 0:
-                    expr += " + cpl%s * intc_shfl(%s, s4%s)" % (ax, V(f1, m + 1), ax)
-                L.append("const double %s = %s;" % (V(t, m), expr))
+                    expr += " + cpl%s * %s(%s, s4%s)" % (ax, shfl, V(f1, m + 1), ax)
+                L.append("const %s %s = %s;" % (R, V(t, m), expr))
     return L, V
 
 
@@ -174,7 +181,12 @@ def _hrr_lines(src, lo, nmove, shift_vec, prefix, kind):
     return L, (lambda a_, b_: names[(a_, b_)])
 
 
-def emit_class_source(cls, spherical):
+def emit_class_source(cls, spherical, prec="d"):
+    """prec "d": FP64 throughout.  prec "f": the per-primitive recurrences (bra
+    chain, ket VRR, cross-lane terms) in FP32 on the Boys values F_m; FP64 for
+    the Boys function, the prefactor exp(-mu R^2) terms, the contraction sums
+    (acc += prefactor * (double) V) and the contracted HRR/transform."""
+    R = "double" if prec == "d" else "float"
     LA, LB, LC, LD = cls
     LX, LR = LA + LB, LC + LD
     M = LX + LR
@@ -186,7 +198,7 @@ def emit_class_source(cls, spherical):
     NCD = nC * nD
     nA = nsph(LA) if spherical else ncc(LA)
     nB = nsph(LB) if spherical else ncc(LB)
-    name = kernel_name(cls, spherical)
+    name = kernel_name(cls, spherical, prec)
     bra_comps = comps_upto(LX)
     ket_comps = comps_upto(LR)
 
@@ -210,8 +222,8 @@ def emit_class_source(cls, spherical):
     o = Emitter()
     o.ind = 0
     o("// Generated by intception_amd/codegen.py -- do not edit.")
-    o("// Canonical class (%d %d | %d %d), %s output, FP64." % (LA, LB, LC, LD,
-                                                                "spherical" if spherical else "Cartesian"))
+    o("// Canonical class (%d %d | %d %d), %s output, %s recurrences." % (
+        LA, LB, LC, LD, "spherical" if spherical else "Cartesian", "FP64" if prec == "d" else "FP32"))
     o('#include "../intc_common.h"')
     o("")
     o("__constant__ int %s_lane[%d][8] = {" % (name, G))
@@ -232,14 +244,17 @@ def emit_class_source(cls, spherical):
     o("if (tw >= TPW) return;")
     o("double *tlds = lds + (size_t)(wave * TPW + tw) * (NE * NCD);")
     o("const int *lt = %s_lane[e];" % name)
-    o("const double ex_d = (double)lt[0], ey_d = (double)lt[1], ez_d = (double)lt[2];")
+    o("const %s ex_d = (%s)lt[0], ey_d = (%s)lt[1], ez_d = (%s)lt[2];" % (R, R, R, R))
     o("const int lbase = lane - e;")
     o("const int s4x = (lbase + lt[3]) * 4, s4y = (lbase + lt[4]) * 4, s4z = (lbase + lt[5]) * 4;")
     o("const int cdir = lt[6], ckk = lt[7];")
     o("(void)ex_d; (void)ey_d; (void)ez_d; (void)s4x; (void)s4y; (void)s4z; (void)cdir; (void)ckk;")
     o("const int nteams = gridDim.x * (blockDim.x >> 6) * TPW;")
-    o("for (int q = (blockIdx.x * (blockDim.x >> 6) + wave) * TPW + tw; q < a.nq; q += nteams) {")
+    o("const int nq = a.nq_dev ? *a.nq_dev : a.nq;")
+    o("for (int qq = (blockIdx.x * (blockDim.x >> 6) + wave) * TPW + tw; qq < nq; qq += nteams) {")
     o.ind = 2
+    o("const int q = a.q_idx ? a.q_idx[qq] : qq;")
+    o("if (q < 0 || q >= a.nq) continue;  // queue entries index the bin")
     o("const int pb = a.q_bra[q], pk = a.q_ket[q];")
     o("const int ob = a.pair_off[pb], Pb = a.pair_np[pb];")
     o("const int okk = a.pair_off[pk], Pk = a.pair_np[pk];")
@@ -268,28 +283,43 @@ def emit_class_source(cls, spherical):
     o("const double pref = kab * kcd * sqrt(os);")
     o("double F[M + 1];")
     o("intc_boys<M, BM>(Tb, F, a);")
+    if prec == "f":
+        # FP32 recurrences on F_m; the prefactor multiplies in FP64 at accumulation
+        o("const float o2zf = (float)o2z, o2ef = (float)o2e;")
     if LX > 0:
         o("const double eo = eta * os;")
-        o("const double PAx = Px - Ax, PAy = Py - Ay, PAz = Pz - Az;")
-        o("const double WPx = -eo * PQx, WPy = -eo * PQy, WPz = -eo * PQz;")
-        o("const double roz = eo;")
+        o("const %s PAx = (%s)(Px - Ax), PAy = (%s)(Py - Ay), PAz = (%s)(Pz - Az);" % (R, R, R, R))
+        o("const %s WPx = (%s)(-eo * PQx), WPy = (%s)(-eo * PQy), WPz = (%s)(-eo * PQz);" % (R, R, R, R))
+        o("const %s roz = (%s)eo;" % (R, R))
     if LR > 0:
         o("const double zo = zeta * os;")
-        o("const double QCx = Qx - Cx, QCy = Qy - Cy, QCz = Qz - Cz;")
-        o("const double WQx = zo * PQx, WQy = zo * PQy, WQz = zo * PQz;")
-        o("const double roe = zo;")
+        o("const %s QCx = (%s)(Qx - Cx), QCy = (%s)(Qy - Cy), QCz = (%s)(Qz - Cz);" % (R, R, R, R))
+        o("const %s WQx = (%s)(zo * PQx), WQy = (%s)(zo * PQy), WQz = (%s)(zo * PQz);" % (R, R, R, R))
+        o("const %s roe = (%s)zo;" % (R, R))
         if LX > 0:
-            o("const double o2s = 0.5 * os;")
-            o("const double cplx = ex_d * o2s, cply = ey_d * o2s, cplz = ez_d * o2s;")
-    for m in range(M + 1):
-        o("const double c0_%d = pref * F[%d];" % (m, m))
-    for ln in _chain_lines(LX, M):
-        o(ln)
-    lines, V = _ket_lines(LX, LR, ncc0(LC - 1))
-    for ln in lines:
-        o(ln)
-    for f in acc_f:
-        o("acc%d += %s;" % (cidx(f), V(f, 0)))
+            o("const %s o2s = (%s)(0.5 * os);" % (R, R))
+            o("const %s cplx = ex_d * o2s, cply = ey_d * o2s, cplz = ez_d * o2s;" % R)
+    if prec == "f":
+        for m in range(M + 1):
+            o("const float c0_%d = (float)F[%d];" % (m, m))
+        body = _chain_lines(LX, M, R)
+        lines, V = _ket_lines(LX, LR, R)
+        body += lines
+        body = [ln.replace("o2z", "o2zf").replace("o2e", "o2ef") for ln in body]
+        for ln in body:
+            o(ln)
+        for f in acc_f:
+            o("acc%d = fma(pref, (double)%s, acc%d);" % (cidx(f), V(f, 0), cidx(f)))
+    else:
+        for m in range(M + 1):
+            o("const double c0_%d = pref * F[%d];" % (m, m))
+        for ln in _chain_lines(LX, M):
+            o(ln)
+        lines, V = _ket_lines(LX, LR)
+        for ln in lines:
+            o(ln)
+        for f in acc_f:
+            o("acc%d += %s;" % (cidx(f), V(f, 0)))
     o.ind = 2
     o("}")
     # team reduction over groups (tree towards group 0)
@@ -386,7 +416,7 @@ def emit_class_source(cls, spherical):
     o("        hipLaunchKernelGGL(%s_refboys, dim3(grid), dim3(block), lds, st, *a);" % name)
     o("    return hipGetLastError();")
     o("}")
-    meta = dict(cls=cls, spherical=spherical, G=G, NE=NE, NCD=NCD, M=M, name=name)
+    meta = dict(cls=cls, spherical=spherical, G=G, NE=NE, NCD=NCD, M=M, name=name, prec=prec)
     return o.text() + "\n", meta
 
 
@@ -402,13 +432,13 @@ def class_set(lmax_pair_side=2, extra=()):
     return sorted(out)
 
 
-def generate(outdir, classes, sph_modes=(False, True)):
+def generate(outdir, classes, sph_modes=(False, True), precs=("d", "f")):
     os.makedirs(outdir, exist_ok=True)
     metas = []
     written = []
     for cls in classes:
-        for sph in sph_modes:
-            src, meta = emit_class_source(cls, sph)
+        for sph, prec in itertools.product(sph_modes, precs):
+            src, meta = emit_class_source(cls, sph, prec)
             fn = os.path.join(outdir, meta["name"] + ".hip")
             old = open(fn).read() if os.path.exists(fn) else None
             if old != src:
@@ -422,12 +452,13 @@ def generate(outdir, classes, sph_modes=(False, True)):
     for m in metas:
         lines.append('extern "C" hipError_t %s_launch(const IntcEriArgs *, int, int, size_t, hipStream_t);'
                      % m["name"])
-    lines.append("struct IntcClassEntry { int la, lb, lc, ld, sph, G, NE, NCD, M; intc_launch_fn fn; };")
+    lines.append("struct IntcClassEntry { int la, lb, lc, ld, sph, prec, G, NE, NCD, M; intc_launch_fn fn; };")
     lines.append("static const IntcClassEntry intc_class_table[] = {")
     for m in metas:
         la, lb, lc, ld = m["cls"]
-        lines.append("    {%d, %d, %d, %d, %d, %d, %d, %d, %d, %s_launch}," % (
-            la, lb, lc, ld, int(m["spherical"]), m["G"], m["NE"], m["NCD"], m["M"], m["name"]))
+        lines.append("    {%d, %d, %d, %d, %d, %d, %d, %d, %d, %d, %s_launch}," % (
+            la, lb, lc, ld, int(m["spherical"]), 0 if m["prec"] == "d" else 1, m["G"], m["NE"], m["NCD"], m["M"],
+            m["name"]))
     lines.append("};")
     lines.append("static const int intc_n_class = %d;" % len(metas))
     tab = os.path.join(outdir, "class_table.inc")

@@ -22,6 +22,8 @@ struct IntcEriArgs {
     const int *q_ket;                 // [nq] ket pair id
     const long long *q_off;           // [nq] output offset (doubles)
     const unsigned long long *q_str;  // [nq] packed strides s_a | s_b<<16 | s_c<<32 | s_d<<48
+    const int *q_idx;                 // optional queue: entry qq works on quartet q_idx[qq]
+    const int *nq_dev;                // optional device-side queue length (overrides nq)
     int nq;
     int team_groups;                  // groups of G lanes cooperating on one quartet
     int boys_mode;                    // 0 Chebyshev, 1 reference three-branch scheme

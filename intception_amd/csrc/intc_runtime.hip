@@ -68,11 +68,11 @@ inline int ncc(int l) { return (l + 1) * (l + 2) / 2; }
 inline int ncc0(int l) { return l < 0 ? 0 : (l + 1) * (l + 2) * (l + 3) / 6; }
 inline int nsph(int l) { return 2 * l + 1; }
 
-const IntcClassEntry *find_class(int la, int lb, int lc, int ld, int sph)
+const IntcClassEntry *find_class(int la, int lb, int lc, int ld, int sph, int prec)
 {
     for (int i = 0; i < intc_n_class; ++i) {
         const IntcClassEntry &c = intc_class_table[i];
-        if (c.la == la && c.lb == lb && c.lc == lc && c.ld == ld && c.sph == sph) return &c;
+        if (c.la == la && c.lb == lb && c.lc == lc && c.ld == ld && c.sph == sph && c.prec == prec) return &c;
     }
     return nullptr;
 }
@@ -145,10 +145,10 @@ int dev_upload(T **d, const std::vector<T> &h)
 }  // namespace
 
 struct IntcBin {
-    const IntcClassEntry *cls;
+    const IntcClassEntry *cls[2];   // [0] FP64 kernel, [1] FP32-recurrence kernel
     int q0, nq;
-    int team_groups;
-    int grid;
+    int team_groups[2];
+    int grid[2];
     int block;
 };
 
@@ -159,20 +159,23 @@ struct intc_plan {
     long long n_pp = 0;
     int n_pair = 0;
     std::vector<IntcBin> bins;
-    // device index data
+    // device index data, in plan (sorted) order
     int *q_bra = nullptr, *q_ket = nullptr;
     long long *q_off = nullptr;
     unsigned long long *q_str = nullptr;
+    int *q_orig = nullptr;        // plan position -> caller's quartet index
+    int *q_bin = nullptr;         // plan position -> bin
+    int *q_size = nullptr;        // integrals in the quartet's block
     int *pair_sa = nullptr, *pair_sb = nullptr, *pair_off = nullptr, *pair_np = nullptr, *pair_nk = nullptr;
     int *pp_pair = nullptr;
     int *shell_off = nullptr;
+    // routed queues: per bin, FP64 and FP32 index lists (segments at q0) and counts
+    int *idx64 = nullptr, *idx32 = nullptr, *counts = nullptr;
+    int *bin_q0 = nullptr;
     // device pair data (filled by execute)
     double *pz = nullptr, *po2z = nullptr, *ppx = nullptr, *ppy = nullptr, *ppz = nullptr, *pkap = nullptr;
     double *pairA = nullptr, *pairAB = nullptr;
-    // host copies for tuning / inspection
-    std::vector<int> bin_prims;
-    // captured execution: pair builder, then every class bin as an independent
-    // graph node (bins fan out over side streams during capture)
+    // captured execution
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     std::vector<hipStream_t> side;
@@ -191,9 +194,10 @@ void plan_free(intc_plan *p)
     p->graph = nullptr;
     p->events.clear();
     p->side.clear();
-    void *ptrs[] = {p->q_bra, p->q_ket, p->q_off, p->q_str, p->pair_sa, p->pair_sb, p->pair_off,
-                    p->pair_np, p->pair_nk, p->pp_pair, p->shell_off, p->pz, p->po2z, p->ppx,
-                    p->ppy, p->ppz, p->pkap, p->pairA, p->pairAB};
+    void *ptrs[] = {p->q_bra, p->q_ket, p->q_off, p->q_str, p->q_orig, p->q_bin, p->q_size, p->pair_sa,
+                    p->pair_sb, p->pair_off, p->pair_np, p->pair_nk, p->pp_pair, p->shell_off, p->idx64,
+                    p->idx32, p->counts, p->bin_q0, p->pz, p->po2z, p->ppx, p->ppy, p->ppz, p->pkap,
+                    p->pairA, p->pairAB};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
 }
@@ -214,17 +218,76 @@ int default_team_groups(const IntcClassEntry *c, int nq, double mean_prims)
     int G = c->G;
     double best = 0.0;
     for (int T = 1; G * T <= 64; ++T) best = std::max(best, lane_use(G, T));
-    int pick = 1, last = 1;
+    int pick = 1;
     for (int T = 1; G * T <= 64; ++T) {
         if (lane_use(G, T) < 0.9 * best) continue;
         if (T > 1 && mean_prims < 2.0 * T) break;
-        last = T;
         double waves = (double)nq * G * T / 64.0;
-        if (waves >= 8.0 * 1024.0) { pick = T; break; }
         pick = T;
+        if (waves >= 8.0 * 1024.0) break;
     }
-    (void)last;
     return pick;
+}
+
+int grid_for(const IntcBin &b, int prec)
+{
+    int tpw = 64 / (b.cls[prec]->G * b.team_groups[prec]);
+    int teams_per_block = (b.block / 64) * tpw;
+    return std::max(1, std::min((b.nq + teams_per_block - 1) / teams_per_block, 256 * 32));
+}
+
+// Route compaction: for every quartet (plan order) read its route code
+// (0 FP64, 1 FP32, 2 skip) and append its position within its bin to the
+// bin's FP64 or FP32 queue; skipped quartets get their output block zeroed.  Positions are
+// claimed per wave (ballot prefix + one atomic per wave and code), so the
+// queues keep roughly the plan's primitive-count order.
+__global__ void __launch_bounds__(256) intc_route_compact(
+    long long nq, const signed char *__restrict__ route, const int *__restrict__ q_orig,
+    const int *__restrict__ q_bin, const int *__restrict__ bin_q0, const long long *__restrict__ q_off,
+    const int *__restrict__ q_size, int *__restrict__ counts, int *__restrict__ idx64, int *__restrict__ idx32,
+    double *__restrict__ out)
+{
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool live = i < nq;
+    int r = 2, b = 0;
+    if (live) {
+        r = route[q_orig[i]];
+        b = q_bin[i];
+        if (r < 0 || r > 2) r = 0;
+    }
+    // a wave's quartets are consecutive in plan order and mostly share a bin;
+    // lanes whose bin differs from lane 0's take the slow path
+    const int lane = threadIdx.x & 63;
+    const int b0 = __shfl(b, 0);
+    const bool same = live && b == b0;
+    for (int code = 0; code < 2; ++code) {
+        const bool want = same && r == code;
+        const unsigned long long m = __ballot(want);
+        if (m == 0ull) continue;
+        const int leader = __ffsll((long long)m) - 1;
+        int base = 0;
+        if (lane == leader) base = atomicAdd(&counts[2 * b0 + code], __popcll(m));
+        base = __shfl(base, leader);
+        if (want) {
+            const int pos = base + __popcll(m & ((1ull << lane) - 1ull));
+            // queue entries are positions relative to the bin's first quartet
+            (code == 0 ? idx64 : idx32)[bin_q0[b0] + pos] = (int)(i - bin_q0[b0]);
+        }
+    }
+    if (live && !same && r < 2) {
+        const int pos = atomicAdd(&counts[2 * b + r], 1);
+        (r == 0 ? idx64 : idx32)[bin_q0[b] + pos] = (int)(i - bin_q0[b]);
+    }
+    if (live && r == 2) {
+        double *o = out + q_off[i];
+        for (int k = 0; k < q_size[i]; ++k) o[k] = 0.0;
+    }
+}
+
+__global__ void intc_zero_ints(int *p, int n)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
 }
 
 }  // namespace
@@ -234,7 +297,7 @@ int default_team_groups(const IntcClassEntry *c, int nq, double mean_prims)
 // ---------------------------------------------------------------------------
 INTC_API const char *intc_last_error(void) { return g_err.c_str(); }
 
-INTC_API int intc_version(void) { return 1; }
+INTC_API int intc_version(void) { return 2; }
 
 INTC_API int intc_set_boys_mode(int mode)
 {
@@ -255,7 +318,7 @@ INTC_API int intc_class_supported(int la, int lb, int lc, int ld, int spherical)
 {
     int l[4] = {la, lb, lc, ld}, cls[4], perm[4];
     canonical(l, cls, perm);
-    return find_class(cls[0], cls[1], cls[2], cls[3], spherical ? 1 : 0) != nullptr;
+    return find_class(cls[0], cls[1], cls[2], cls[3], spherical ? 1 : 0, 0) != nullptr;
 }
 
 INTC_API int intc_plan_create(intc_plan **out, int spherical, int n_shell, const int *shell_l,
@@ -265,6 +328,7 @@ INTC_API int intc_plan_create(intc_plan **out, int spherical, int n_shell, const
     (void)shell_cen;
     *out = nullptr;
     if (n_shell < 0 || n_quartet < 0) return fail(-1, "negative size");
+    if (n_quartet > 0x7fffffffLL) return fail(-1, "too many quartets for one plan");
     for (int s = 0; s < n_shell; ++s) {
         if (shell_l[s] < 0 || shell_l[s] > 7) return fail(-1, "shell angular momentum out of range");
         if (shell_nprim[s] < 1 || shell_nprim[s] > 64) return fail(-1, "shell primitive count out of range");
@@ -272,12 +336,9 @@ INTC_API int intc_plan_create(intc_plan **out, int spherical, int n_shell, const
     intc_plan *p = new intc_plan();
     p->spherical = spherical ? 1 : 0;
     HIPCHK(hipGetDevice(&p->device));
-    // canonical class and permutation per quartet
     const int nq = (int)n_quartet;
-    std::vector<int> key(nq);
     std::vector<const IntcClassEntry *> entries;
-    std::vector<int> cls_of(nq);
-    std::vector<int> prims(nq);
+    std::vector<int> cls_of(nq), prims(nq);
     std::vector<std::array<int, 4>> perms(nq);
     for (int q = 0; q < nq; ++q) {
         int l[4], cls[4], perm[4];
@@ -287,7 +348,7 @@ INTC_API int intc_plan_create(intc_plan **out, int spherical, int n_shell, const
             l[k] = shell_l[s];
         }
         canonical(l, cls, perm);
-        const IntcClassEntry *e = find_class(cls[0], cls[1], cls[2], cls[3], p->spherical);
+        const IntcClassEntry *e = find_class(cls[0], cls[1], cls[2], cls[3], p->spherical, 0);
         if (!e) {
             plan_free(p);
             delete p;
@@ -316,7 +377,8 @@ INTC_API int intc_plan_create(intc_plan **out, int spherical, int n_shell, const
         return x < y;
     });
     // pairs: one bra and one ket pair per quartet (canonical shell order)
-    std::vector<int> q_bra(nq), q_ket(nq), pair_sa(2 * (size_t)nq), pair_sb(2 * (size_t)nq);
+    std::vector<int> q_bra(nq), q_ket(nq), q_size(nq), q_bin(nq), pair_sa(2 * (size_t)nq),
+        pair_sb(2 * (size_t)nq);
     std::vector<int> pair_off(2 * (size_t)nq), pair_np(2 * (size_t)nq), pair_nk(2 * (size_t)nq);
     std::vector<long long> q_off(nq);
     std::vector<unsigned long long> q_str(nq);
@@ -339,12 +401,12 @@ INTC_API int intc_plan_create(intc_plan **out, int spherical, int n_shell, const
         q_bra[i] = 2 * i;
         q_ket[i] = 2 * i + 1;
         q_off[i] = out_off[q];
-        // original row-major strides
         int n[4];
         for (int k = 0; k < 4; ++k) {
             int l = shell_l[s[k]];
             n[k] = p->spherical ? nsph(l) : ncc(l);
         }
+        q_size[i] = n[0] * n[1] * n[2] * n[3];
         int st[4] = {n[1] * n[2] * n[3], n[2] * n[3], n[3], 1};
         unsigned long long pk = 0;
         for (int k = 0; k < 4; ++k) pk |= (unsigned long long)(st[pm[k]] & 0xffff) << (16 * k);
@@ -355,20 +417,25 @@ INTC_API int intc_plan_create(intc_plan **out, int spherical, int n_shell, const
     for (size_t pr = 0; pr < pair_off.size(); ++pr)
         for (int k = 0; k < pair_np[pr]; ++k) pp_pair[pair_off[pr] + k] = (int)pr;
     // bins
+    std::vector<int> bin_q0;
     for (int i = 0; i < nq;) {
         int j = i;
         int c = cls_of[order[i]];
         double sum = 0.0;
         while (j < nq && cls_of[order[j]] == c) sum += prims[order[j++]];
         IntcBin b;
-        b.cls = entries[c];
+        b.cls[0] = entries[c];
+        b.cls[1] = find_class(b.cls[0]->la, b.cls[0]->lb, b.cls[0]->lc, b.cls[0]->ld, p->spherical, 1);
+        if (!b.cls[1]) b.cls[1] = b.cls[0];
         b.q0 = i;
         b.nq = j - i;
-        b.team_groups = default_team_groups(b.cls, j - i, sum / (j - i));
         b.block = 256;
-        int tpw = 64 / (b.cls->G * b.team_groups);
-        int teams_per_block = (b.block / 64) * tpw;
-        b.grid = std::min((b.nq + teams_per_block - 1) / teams_per_block, 256 * 32);
+        for (int pr = 0; pr < 2; ++pr) {
+            b.team_groups[pr] = default_team_groups(b.cls[pr], j - i, sum / (j - i));
+            b.grid[pr] = grid_for(b, pr);
+        }
+        for (int k = i; k < j; ++k) q_bin[k] = (int)p->bins.size();
+        bin_q0.push_back(i);
         p->bins.push_back(b);
         i = j;
     }
@@ -378,6 +445,10 @@ INTC_API int intc_plan_create(intc_plan **out, int spherical, int n_shell, const
     rc |= dev_upload(&p->q_ket, q_ket);
     rc |= dev_upload(&p->q_off, q_off);
     rc |= dev_upload(&p->q_str, q_str);
+    rc |= dev_upload(&p->q_orig, order);
+    rc |= dev_upload(&p->q_bin, q_bin);
+    rc |= dev_upload(&p->q_size, q_size);
+    rc |= dev_upload(&p->bin_q0, bin_q0);
     rc |= dev_upload(&p->pair_sa, pair_sa);
     rc |= dev_upload(&p->pair_sb, pair_sb);
     rc |= dev_upload(&p->pair_off, pair_off);
@@ -390,6 +461,15 @@ INTC_API int intc_plan_create(intc_plan **out, int spherical, int n_shell, const
     p->n_pair = (int)npair;
     p->n_pp = npp;
     p->n_quartet = nq;
+    if (nq > 0) {
+        if (hipMalloc(&p->idx64, sizeof(int) * nq) != hipSuccess ||
+            hipMalloc(&p->idx32, sizeof(int) * nq) != hipSuccess ||
+            hipMalloc(&p->counts, sizeof(int) * 2 * std::max<size_t>(1, p->bins.size())) != hipSuccess) {
+            plan_free(p);
+            delete p;
+            return fail(-3, "hipMalloc route queues");
+        }
+    }
     if (npp > 0) {
         double **arrs[] = {&p->pz, &p->po2z, &p->ppx, &p->ppy, &p->ppz, &p->pkap};
         for (double **a : arrs) {
@@ -409,42 +489,49 @@ INTC_API int intc_plan_create(intc_plan **out, int spherical, int n_shell, const
 
 INTC_API int intc_plan_num_bins(const intc_plan *p) { return (int)p->bins.size(); }
 
-INTC_API int intc_plan_bin_info(const intc_plan *p, int i, int *cls4, int *nq, int *team_groups, int *grid,
-                                int *G)
+INTC_API int intc_plan_bin_info(const intc_plan *p, int i, int prec, int *cls4, int *nq, int *team_groups,
+                                int *grid, int *G)
 {
     if (i < 0 || i >= (int)p->bins.size()) return fail(-1, "bin index out of range");
+    if (prec != 0 && prec != 1) return fail(-1, "prec must be 0 (FP64) or 1 (FP32 recurrences)");
     const IntcBin &b = p->bins[i];
-    cls4[0] = b.cls->la; cls4[1] = b.cls->lb; cls4[2] = b.cls->lc; cls4[3] = b.cls->ld;
+    cls4[0] = b.cls[0]->la; cls4[1] = b.cls[0]->lb; cls4[2] = b.cls[0]->lc; cls4[3] = b.cls[0]->ld;
     *nq = b.nq;
-    *team_groups = b.team_groups;
-    *grid = b.grid;
-    *G = b.cls->G;
+    *team_groups = b.team_groups[prec];
+    *grid = b.grid[prec];
+    *G = b.cls[prec]->G;
     return 0;
 }
 
-INTC_API int intc_plan_set_bin_tuning(intc_plan *p, int i, int team_groups, int grid)
+INTC_API int intc_plan_set_bin_tuning(intc_plan *p, int i, int prec, int team_groups, int grid)
 {
     if (i < 0 || i >= (int)p->bins.size()) return fail(-1, "bin index out of range");
+    if (prec != 0 && prec != 1) return fail(-1, "prec must be 0 (FP64) or 1 (FP32 recurrences)");
     IntcBin &b = p->bins[i];
-    if (team_groups < 1 || team_groups * b.cls->G > 64) return fail(-1, "team_groups * G must be in [1, 64]");
+    if (team_groups < 1 || team_groups * b.cls[prec]->G > 64) return fail(-1, "team_groups * G must be in [1, 64]");
     if (grid < 1) return fail(-1, "grid must be positive");
-    b.team_groups = team_groups;
-    b.grid = grid;
+    b.team_groups[prec] = team_groups;
+    b.grid[prec] = grid;
     return 0;
 }
 
-static int launch_bin(const intc_plan *p, const IntcBin &b, const IntcEriArgs &base, hipStream_t st)
+// launch one bin in one precision; q_idx/nq_dev select a routed queue
+static int launch_bin(const intc_plan *p, const IntcBin &b, int prec, const IntcEriArgs &base, const int *q_idx,
+                      const int *nq_dev, hipStream_t st)
 {
     IntcEriArgs a = base;
     a.q_bra = p->q_bra + b.q0;
     a.q_ket = p->q_ket + b.q0;
     a.q_off = p->q_off + b.q0;
     a.q_str = p->q_str + b.q0;
+    a.q_idx = q_idx;
+    a.nq_dev = nq_dev;
     a.nq = b.nq;
-    a.team_groups = b.team_groups;
-    int tpw = 64 / (b.cls->G * b.team_groups);
-    size_t lds = (size_t)(b.block / 64) * tpw * b.cls->NE * b.cls->NCD * sizeof(double);
-    hipError_t e = b.cls->fn(&a, b.grid, b.block, lds, st);
+    a.team_groups = b.team_groups[prec];
+    const IntcClassEntry *c = b.cls[prec];
+    int tpw = 64 / (c->G * a.team_groups);
+    size_t lds = (size_t)(b.block / 64) * tpw * c->NE * c->NCD * sizeof(double);
+    hipError_t e = c->fn(&a, b.grid[prec], b.block, lds, st);
     if (e != hipSuccess) return fail(-3, std::string("class kernel launch: ") + hipGetErrorString(e));
     return 0;
 }
@@ -488,14 +575,87 @@ INTC_API int intc_plan_build_pairs(intc_plan *p, const double *exps, const doubl
     return 0;
 }
 
-INTC_API int intc_plan_run_bin(intc_plan *p, int i, const double *sph, double *out, void *stream)
+INTC_API int intc_plan_run_bin(intc_plan *p, int i, int prec, const double *sph, double *out, void *stream)
 {
     if (i < 0 || i >= (int)p->bins.size()) return fail(-1, "bin index out of range");
+    if (prec != 0 && prec != 1) return fail(-1, "prec must be 0 (FP64) or 1 (FP32 recurrences)");
     if (p->spherical && !sph) return fail(-1, "spherical plan needs the S matrices");
     IntcEriArgs a;
     int rc = fill_args(p, sph, out, &a);
     if (rc) return rc;
-    return launch_bin(p, p->bins[i], a, (hipStream_t)stream);
+    return launch_bin(p, p->bins[i], prec, a, nullptr, nullptr, (hipStream_t)stream);
+}
+
+// device route: one int8 per caller quartet (0 FP64, 1 FP32, 2 skip)
+static int route_queues(intc_plan *p, const signed char *route, double *out, hipStream_t st)
+{
+    int nb = (int)p->bins.size();
+    if (p->n_quartet == 0) return 0;
+    hipLaunchKernelGGL(intc_zero_ints, dim3((2 * nb + 255) / 256), dim3(256), 0, st, p->counts, 2 * nb);
+    HIPCHK(hipGetLastError());
+    int blocks = (int)((p->n_quartet + 255) / 256);
+    hipLaunchKernelGGL(intc_route_compact, dim3(blocks), dim3(256), 0, st, p->n_quartet, route, p->q_orig,
+                       p->q_bin, p->bin_q0, p->q_off, p->q_size, p->counts, p->idx64, p->idx32, out);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+static int run_all(intc_plan *p, const double *sph, double *out, const signed char *route, int n_streams,
+                   hipStream_t st)
+{
+    IntcEriArgs a;
+    int rc = fill_args(p, sph, out, &a);
+    if (rc) return rc;
+    if (route) {
+        rc = route_queues(p, route, out, st);
+        if (rc) return rc;
+    }
+    if (n_streams > 1) {
+        if (hipEventRecord(p->events[0], st) != hipSuccess) return fail(-3, "event record");
+        for (int k = 0; k < n_streams - 1; ++k)
+            if (hipStreamWaitEvent(p->side[k], p->events[0], 0) != hipSuccess) return fail(-3, "stream wait");
+    }
+    // largest bins first, round-robin over the streams
+    std::vector<int> ord(p->bins.size());
+    for (size_t i = 0; i < ord.size(); ++i) ord[i] = (int)i;
+    std::sort(ord.begin(), ord.end(), [&](int x, int y) {
+        return (double)p->bins[x].nq * p->bins[x].cls[0]->NE * p->bins[x].cls[0]->NCD >
+               (double)p->bins[y].nq * p->bins[y].cls[0]->NE * p->bins[y].cls[0]->NCD;
+    });
+    int k = 0;
+    for (int bi : ord) {
+        const IntcBin &b = p->bins[bi];
+        for (int prec = 0; prec < (route ? 2 : 1); ++prec) {
+            hipStream_t s2 = (k % n_streams) == 0 ? st : p->side[(k % n_streams) - 1];
+            ++k;
+            const int *qi = route ? (prec == 0 ? p->idx64 : p->idx32) + b.q0 : nullptr;
+            const int *nd = route ? p->counts + 2 * bi + prec : nullptr;
+            rc = launch_bin(p, b, prec, a, qi, nd, s2);
+            if (rc) return rc;
+        }
+    }
+    if (n_streams > 1) {
+        for (int j = 0; j < n_streams - 1; ++j) {
+            if (hipEventRecord(p->events[j + 1], p->side[j]) != hipSuccess) return fail(-3, "event record");
+            if (hipStreamWaitEvent(st, p->events[j + 1], 0) != hipSuccess) return fail(-3, "stream wait");
+        }
+    }
+    return 0;
+}
+
+static int ensure_streams(intc_plan *p, int n_streams)
+{
+    while ((int)p->side.size() < n_streams - 1) {
+        hipStream_t s2;
+        HIPCHK(hipStreamCreateWithFlags(&s2, hipStreamNonBlocking));
+        p->side.push_back(s2);
+    }
+    while ((int)p->events.size() < n_streams) {
+        hipEvent_t e;
+        HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        p->events.push_back(e);
+    }
+    return 0;
 }
 
 INTC_API int intc_plan_execute(intc_plan *p, const double *exps, const double *coefs, const double *shell_cen,
@@ -504,22 +664,27 @@ INTC_API int intc_plan_execute(intc_plan *p, const double *exps, const double *c
     if (p->spherical && !sph) return fail(-1, "spherical plan needs the S matrices");
     int rc = intc_plan_build_pairs(p, exps, coefs, shell_cen, stream);
     if (rc) return rc;
-    IntcEriArgs a;
-    rc = fill_args(p, sph, out, &a);
-    if (rc) return rc;
-    for (const IntcBin &b : p->bins) {
-        rc = launch_bin(p, b, a, (hipStream_t)stream);
-        if (rc) return rc;
-    }
-    return 0;
+    return run_all(p, sph, out, nullptr, 1, (hipStream_t)stream);
 }
 
-// Capture pair building and all class bins into one hipGraph.  With
-// n_streams > 1 the bins are spread over side streams forked from `stream`,
-// so the graph has one pair-builder node followed by independent bin nodes
-// that the device runs concurrently; n_streams = 1 keeps them in sequence.
+INTC_API int intc_plan_execute_routed(intc_plan *p, const double *exps, const double *coefs,
+                                      const double *shell_cen, const double *sph, const signed char *route,
+                                      double *out, void *stream)
+{
+    if (p->spherical && !sph) return fail(-1, "spherical plan needs the S matrices");
+    if (!route) return fail(-1, "route array is required");
+    int rc = intc_plan_build_pairs(p, exps, coefs, shell_cen, stream);
+    if (rc) return rc;
+    return run_all(p, sph, out, route, 1, (hipStream_t)stream);
+}
+
+// Capture pair building, the optional route compaction and all class bins
+// into one hipGraph.  With n_streams > 1 the bins are spread over side
+// streams forked from `stream`, so the graph has independent bin nodes that
+// the device runs concurrently.
 INTC_API int intc_plan_graph_create(intc_plan *p, const double *exps, const double *coefs, const double *shell_cen,
-                                    const double *sph, double *out, int n_streams, void *stream)
+                                    const double *sph, const signed char *route, double *out, int n_streams,
+                                    void *stream)
 {
     if (n_streams < 1 || n_streams > 16) return fail(-1, "n_streams must be in [1, 16]");
     if (p->spherical && !sph) return fail(-1, "spherical plan needs the S matrices");
@@ -531,50 +696,22 @@ INTC_API int intc_plan_graph_create(intc_plan *p, const double *exps, const doub
         (void)hipGraphDestroy(p->graph);
         p->graph = nullptr;
     }
+    int rc = ensure_streams(p, n_streams);
+    if (rc) return rc;
     hipStream_t st = (hipStream_t)stream;
     bool own = false;
     if (!st) {
         HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
         own = true;
     }
-    while ((int)p->side.size() < n_streams - 1) {
-        hipStream_t s2;
-        HIPCHK(hipStreamCreateWithFlags(&s2, hipStreamNonBlocking));
-        p->side.push_back(s2);
+    {
+        DevTables *t;
+        rc = ensure_tables(p->device, &t);
+        if (rc) return rc;
     }
-    while ((int)p->events.size() < n_streams) {
-        hipEvent_t e;
-        HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-        p->events.push_back(e);
-    }
-    IntcEriArgs a;
-    int rc = fill_args(p, sph, out, &a);
-    if (rc) return rc;
     HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
     rc = intc_plan_build_pairs(p, exps, coefs, shell_cen, st);
-    if (!rc && n_streams > 1) {
-        if (hipEventRecord(p->events[0], st) != hipSuccess) rc = fail(-3, "event record");
-        for (int k = 0; !rc && k < n_streams - 1; ++k)
-            if (hipStreamWaitEvent(p->side[k], p->events[0], 0) != hipSuccess) rc = fail(-3, "stream wait");
-    }
-    // largest bins first, round-robin over the streams
-    std::vector<int> ord(p->bins.size());
-    for (size_t i = 0; i < ord.size(); ++i) ord[i] = (int)i;
-    std::sort(ord.begin(), ord.end(), [&](int x, int y) {
-        return (double)p->bins[x].nq * p->bins[x].cls->NE * p->bins[x].cls->NCD >
-               (double)p->bins[y].nq * p->bins[y].cls->NE * p->bins[y].cls->NCD;
-    });
-    for (size_t k = 0; !rc && k < ord.size(); ++k) {
-        int sidx = (int)(k % n_streams);
-        hipStream_t s2 = sidx == 0 ? st : p->side[sidx - 1];
-        rc = launch_bin(p, p->bins[ord[k]], a, s2);
-    }
-    if (!rc && n_streams > 1) {
-        for (int k = 0; !rc && k < n_streams - 1; ++k) {
-            if (hipEventRecord(p->events[k + 1], p->side[k]) != hipSuccess) rc = fail(-3, "event record");
-            else if (hipStreamWaitEvent(st, p->events[k + 1], 0) != hipSuccess) rc = fail(-3, "stream wait");
-        }
-    }
+    if (!rc) rc = run_all(p, sph, out, route, n_streams, st);
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(st, &g);
     if (own) (void)hipStreamDestroy(st);
@@ -592,6 +729,15 @@ INTC_API int intc_plan_graph_launch(intc_plan *p, void *stream)
 {
     if (!p->graph_exec) return fail(-1, "no graph captured for this plan");
     HIPCHK(hipGraphLaunch(p->graph_exec, (hipStream_t)stream));
+    return 0;
+}
+
+// queue lengths of the last routed execution (host copy, synchronising)
+INTC_API int intc_plan_route_counts(intc_plan *p, int *counts_out)
+{
+    int nb = (int)p->bins.size();
+    if (nb == 0) return 0;
+    HIPCHK(hipMemcpy(counts_out, p->counts, sizeof(int) * 2 * nb, hipMemcpyDeviceToHost));
     return 0;
 }
 

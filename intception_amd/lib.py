@@ -31,14 +31,16 @@ SIGNATURES = [
     ("intc_eri_cont_cart_main", c_int, _SHELL * 4 + [_dp, _dp]),
     ("intc_plan_create", c_int, [ctypes.POINTER(c_void_p), c_int, c_int, _ip, _ip, _ip, _dp, c_ll, _ip, _llp]),
     ("intc_plan_num_bins", c_int, [c_void_p]),
-    ("intc_plan_bin_info", c_int, [c_void_p, c_int, _ip, _ip, _ip, _ip, _ip]),
-    ("intc_plan_set_bin_tuning", c_int, [c_void_p, c_int, c_int, c_int]),
+    ("intc_plan_bin_info", c_int, [c_void_p, c_int, c_int, _ip, _ip, _ip, _ip, _ip]),
+    ("intc_plan_set_bin_tuning", c_int, [c_void_p, c_int, c_int, c_int, c_int]),
     ("intc_plan_build_pairs", c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
-    ("intc_plan_run_bin", c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
+    ("intc_plan_run_bin", c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p]),
     ("intc_plan_execute", c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
-    ("intc_plan_graph_create", c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int,
-                                       c_void_p]),
+    ("intc_plan_execute_routed", c_int, [c_void_p] * 8),
+    ("intc_plan_graph_create", c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                       c_int, c_void_p]),
     ("intc_plan_graph_launch", c_int, [c_void_p, c_void_p]),
+    ("intc_plan_route_counts", c_int, [c_void_p, _ip]),
     ("intc_plan_destroy", c_int, [c_void_p]),
 ]
 

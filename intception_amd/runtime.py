@@ -61,21 +61,22 @@ class EriPlan:
         self.d_sph = torch.from_numpy(sph_blob()).to(self.device) if b.spherical else None
         self.out_size = int(b.out_size)
 
-    def bins(self):
+    def bins(self, prec=0):
+        """Class bins; prec 0 = FP64 kernels, 1 = FP32-recurrence kernels."""
         L = self._L
         n = L.intc_plan_num_bins(self.handle)
         out = []
         for i in range(n):
             cls = (ctypes.c_int * 4)()
             nq, T, grid, G = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
-            _lib.check(L.intc_plan_bin_info(self.handle, i, cls, ctypes.byref(nq), ctypes.byref(T),
+            _lib.check(L.intc_plan_bin_info(self.handle, i, int(prec), cls, ctypes.byref(nq), ctypes.byref(T),
                                             ctypes.byref(grid), ctypes.byref(G)))
             out.append({"bin": i, "cls": tuple(cls), "nq": nq.value, "team_groups": T.value,
-                        "grid": grid.value, "G": G.value})
+                        "grid": grid.value, "G": G.value, "prec": prec})
         return out
 
-    def set_tuning(self, i, team_groups, grid):
-        _lib.check(self._L.intc_plan_set_bin_tuning(self.handle, int(i), int(team_groups), int(grid)))
+    def set_tuning(self, i, team_groups, grid, prec=0):
+        _lib.check(self._L.intc_plan_set_bin_tuning(self.handle, int(i), int(prec), int(team_groups), int(grid)))
 
     def _stream(self):
         torch = _torch()
@@ -89,9 +90,9 @@ class EriPlan:
         _lib.check(self._L.intc_plan_build_pairs(self.handle, _ptr(self.d_exps), _ptr(self.d_coefs),
                                                  _ptr(self.d_cen), self._stream()))
 
-    def run_bin(self, i, out):
+    def run_bin(self, i, out, prec=0):
         sph = _ptr(self.d_sph) if self.d_sph is not None else ctypes.c_void_p()
-        _lib.check(self._L.intc_plan_run_bin(self.handle, int(i), sph, _ptr(out), self._stream()))
+        _lib.check(self._L.intc_plan_run_bin(self.handle, int(i), int(prec), sph, _ptr(out), self._stream()))
 
     def run(self, out=None):
         if out is None:
@@ -101,11 +102,28 @@ class EriPlan:
                                              _ptr(self.d_cen), sph, _ptr(out), self._stream()))
         return out
 
-    def capture(self, out, n_streams=4):
-        """Capture pair building + all bins into one hipGraph bound to `out`."""
+    def run_routed(self, route, out=None):
+        """route: device int8 tensor, one code per quartet (0 FP64, 1 FP32, 2 skip)."""
+        if out is None:
+            out = self.alloc_out()
         sph = _ptr(self.d_sph) if self.d_sph is not None else ctypes.c_void_p()
+        _lib.check(self._L.intc_plan_execute_routed(self.handle, _ptr(self.d_exps), _ptr(self.d_coefs),
+                                                    _ptr(self.d_cen), sph, _ptr(route), _ptr(out), self._stream()))
+        return out
+
+    def route_counts(self):
+        n = self._L.intc_plan_num_bins(self.handle)
+        buf = (ctypes.c_int * (2 * max(n, 1)))()
+        _lib.check(self._L.intc_plan_route_counts(self.handle, buf))
+        return np.array(buf[:2 * n]).reshape(n, 2)
+
+    def capture(self, out, n_streams=4, route=None):
+        """Capture pair building (+ route compaction) + all bins into one
+        hipGraph bound to `out` (and `route`)."""
+        sph = _ptr(self.d_sph) if self.d_sph is not None else ctypes.c_void_p()
+        r = _ptr(route) if route is not None else ctypes.c_void_p()
         _lib.check(self._L.intc_plan_graph_create(self.handle, _ptr(self.d_exps), _ptr(self.d_coefs),
-                                                  _ptr(self.d_cen), sph, _ptr(out), int(n_streams),
+                                                  _ptr(self.d_cen), sph, r, _ptr(out), int(n_streams),
                                                   self._stream()))
         self._graph_out = out
         return out

@@ -52,3 +52,36 @@ def test_all_81_classes_each_permutation(gpu):
         b = basis.make_random_quartets(4242 + sph, np.repeat(cls, 3, axis=0), [1, 2, 3], sph)
         print("all classes, spherical=%s: %d quartets" % (sph, b.n_quartet), flush=True)
         assert_parity(_gpu(b, 0), _oracle(b, 0), what="81 classes sph=%s" % sph)
+
+
+def _golden3c():
+    import os
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "eri3c_mp_golden.npz"))
+    qs = []
+    for q in range(len(g["l"])):
+        qs.append([{"l": int(g["l"][q][k]), "exps": g["exps"][q][k][:g["nprim"][q][k]],
+                    "coefs": g["coefs"][q][k][:g["nprim"][q][k]], "cen": g["cen"][q][k]} for k in range(4)])
+    return basis.batch_from_shells(qs, True), g["val"]
+
+
+def test_three_center_golden(gpu):
+    # (ab|P s), unit s-shell of exponent 0, against the 40-digit MD fixtures
+    b, ref = _golden3c()
+    assert_parity(_gpu(b, 0), ref, what="three-centre vs MD")
+
+
+def test_three_center_classes(gpu):
+    # (ab|P s) with a unit s-shell: orbital l <= 2 (K 1..6), auxiliary l <= 4.
+    # The oracle (the reference's a->c->d VRR scheme) loses up to ~1.1e-11 of
+    # the block maximum to rounding on these classes (diffuse auxiliary
+    # exponents, centres up to 17 bohr apart; measured against the MD
+    # fixtures in tests/test_oracle.py), so the bar adds 5e-11 x block max.
+    b = basis.random_three_center(31337, 3000)
+    print("three-centre: %d quartets" % b.n_quartet, flush=True)
+    got, ref = _gpu(b, 0), _oracle(b, 0)
+    sz = b.sizes()
+    for q in range(b.n_quartet):
+        o, n = b.out_off[q], sz[q]
+        g, r = got[o:o + n], ref[o:o + n]
+        lim = 1e-14 + 1e-12 * np.abs(r) + 5e-11 * np.abs(r).max()
+        assert np.all(np.abs(g - r) <= lim), (q, np.abs(g - r).max(), np.abs(r).max())
