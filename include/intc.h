@@ -82,6 +82,13 @@ typedef struct intc_plan intc_plan;
 int intc_plan_create(intc_plan **plan, int spherical, int n_shell, const int *shell_l, const int *shell_nprim,
                      const int *shell_off, const double *shell_cen, long long n_quartet, const int *quartets,
                      const long long *out_off);
+/* Same, with shell-pair data shared between quartets: pair P is shells
+ * (pair_shells[2P], pair_shells[2P+1]); quartet q is (pair quartet_pairs[2q] |
+ * pair quartet_pairs[2q+1]) with a row-major block over those four shells in
+ * that order at out[out_off[q]]. */
+int intc_plan_create_pairs(intc_plan **plan, int spherical, int n_shell, const int *shell_l, const int *shell_nprim,
+                           const int *shell_off, int n_pair, const int *pair_shells, long long n_quartet,
+                           const int *quartet_pairs, const long long *out_off);
 int intc_plan_num_bins(const intc_plan *plan);
 /* prec: 0 = FP64 kernel, 1 = kernel with FP32 recurrences (FP64 Boys,
  * prefactors, contraction sums and contracted HRR/transform). */

@@ -132,6 +132,10 @@ class QuartetBatch:
     out_off: np.ndarray = None
     out_size: int = 0
     meta: dict = field(default_factory=dict)
+    # optional shared shell pairs: pair_shells (n_pair, 2), quartet_pairs
+    # (n_quartet, 2); then quartets[q] = pair_shells[quartet_pairs[q]].ravel()
+    pair_shells: np.ndarray = None
+    quartet_pairs: np.ndarray = None
 
     def __post_init__(self):
         if self.out_off is None:
@@ -169,9 +173,11 @@ class QuartetBatch:
 
     def subset(self, idx):
         """Batch with the given quartets (same shell table, dense output)."""
+        qp = None if self.quartet_pairs is None else np.ascontiguousarray(self.quartet_pairs[idx])
         return QuartetBatch(self.spherical, self.shell_l, self.shell_nprim, self.shell_off,
                             self.shell_cen, self.exps, self.coefs,
-                            np.ascontiguousarray(self.quartets[idx]), meta=dict(self.meta))
+                            np.ascontiguousarray(self.quartets[idx]), meta=dict(self.meta),
+                            pair_shells=self.pair_shells, quartet_pairs=qp)
 
 
 def random_shells(rng, ls, nprims, box=10.0, log_alpha=(-1.0, 2.0)):

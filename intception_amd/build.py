@@ -66,6 +66,7 @@ def build(lmax=2, jobs=None, verbose=True):
     srcs = [(f, common) for f in files]
     srcs.append((os.path.join(CSRC, "intc_runtime.hip"),
                  common + [inc, os.path.join(GEN, "class_table.inc")]))
+    srcs.append((os.path.join(CSRC, "intc_mlp.hip"), []))
     jobs = jobs or min(16, os.cpu_count() or 4)
     with ThreadPoolExecutor(jobs) as ex:
         res = list(ex.map(lambda a: _compile(*a), srcs))

@@ -284,24 +284,33 @@ def emit_class_source(cls, spherical, prec="d"):
     o("double F[M + 1];")
     o("intc_boys<M, BM>(Tb, F, a);")
     if prec == "f":
-        # FP32 recurrences on F_m; the prefactor multiplies in FP64 at accumulation
+        # FP32 recurrences on s^m F_m with s = max(1, 2T): F_m(T) ~ (2m-1)!! /
+        # (2T)^m F_0 underflows FP32 for large T, s^m F_m stays O(F_0).  Every
+        # term that raises m by one carries 1/s; results at m = 0 are unscaled.
+        # The prefactor multiplies in FP64 at accumulation.
         o("const float o2zf = (float)o2z, o2ef = (float)o2e;")
+        o("const double sc = Tb > 0.5 ? 2.0 * Tb : 1.0, isc = 1.0 / sc;")
+    else:
+        o("const double isc = 1.0;")
     if LX > 0:
         o("const double eo = eta * os;")
         o("const %s PAx = (%s)(Px - Ax), PAy = (%s)(Py - Ay), PAz = (%s)(Pz - Az);" % (R, R, R, R))
-        o("const %s WPx = (%s)(-eo * PQx), WPy = (%s)(-eo * PQy), WPz = (%s)(-eo * PQz);" % (R, R, R, R))
-        o("const %s roz = (%s)eo;" % (R, R))
+        o("const %s WPx = (%s)(-eo * PQx * isc), WPy = (%s)(-eo * PQy * isc), WPz = (%s)(-eo * PQz * isc);"
+          % (R, R, R, R))
+        o("const %s roz = (%s)(eo * isc);" % (R, R))
     if LR > 0:
         o("const double zo = zeta * os;")
         o("const %s QCx = (%s)(Qx - Cx), QCy = (%s)(Qy - Cy), QCz = (%s)(Qz - Cz);" % (R, R, R, R))
-        o("const %s WQx = (%s)(zo * PQx), WQy = (%s)(zo * PQy), WQz = (%s)(zo * PQz);" % (R, R, R, R))
-        o("const %s roe = (%s)zo;" % (R, R))
+        o("const %s WQx = (%s)(zo * PQx * isc), WQy = (%s)(zo * PQy * isc), WQz = (%s)(zo * PQz * isc);"
+          % (R, R, R, R))
+        o("const %s roe = (%s)(zo * isc);" % (R, R))
         if LX > 0:
-            o("const %s o2s = (%s)(0.5 * os);" % (R, R))
+            o("const %s o2s = (%s)(0.5 * os * isc);" % (R, R))
             o("const %s cplx = ex_d * o2s, cply = ey_d * o2s, cplz = ez_d * o2s;" % R)
     if prec == "f":
+        o("double scm = 1.0;")
         for m in range(M + 1):
-            o("const float c0_%d = (float)F[%d];" % (m, m))
+            o("const float c0_%d = (float)(F[%d] * scm);%s" % (m, m, " scm *= sc;" if m < M else ""))
         body = _chain_lines(LX, M, R)
         lines, V = _ket_lines(LX, LR, R)
         body += lines

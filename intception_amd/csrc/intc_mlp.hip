@@ -1,0 +1,363 @@
+// intc_mlp.hip -- the quartet-magnitude network: 12 -> 64 -> 64 -> 1, ReLU.
+//
+// Predicts log10(max |integral|) of a contracted quartet from pair features,
+// to tighten Schwarz screening (skip) and to route quartets between the FP64
+// and FP32-recurrence kernels.  Weights are stored as BF16 for inference
+// (FP32 master copies for training); every product accumulates in FP32.
+//
+//   intc_mlp_route   features (from the pair table) -> 3 layers on
+//                    v_mfma_f32_16x16x16_bf16 -> route code per quartet
+//   intc_mlp_train   one minibatch: forward + backward per sample (VALU FP32
+//                    with the BF16-rounded weights), per-block partial
+//                    gradients (deterministic, no atomics)
+//   intc_mlp_adam    sums the partials and applies Adam to the master weights,
+//                    refreshing the BF16 copies
+//
+// Pair feature table (8 floats per shell pair): log10 Q (Schwarz factor),
+// log10 zeta_min, |AB|, la + lb, P_x, P_y, P_z (centre of the most diffuse
+// primitive pair), zeta_min.  Quartet features (12, standardised with the
+// model's mean/scale): log10 Q_bra, log10 Q_ket, log10 zeta_bra, log10 zeta_ket,
+// |AB|, |CD|, L_bra, L_ket, R_PQ, log10 rho, log10(1 + rho R_PQ^2),
+// log10 Q_bra + log10 Q_ket.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <algorithm>
+#include <cmath>
+
+#define INTC_API extern "C" __attribute__((visibility("default")))
+
+namespace {
+
+constexpr int NF = 12, KF = 16, NH = 64;
+// FP32 master parameter layout
+constexpr int OW1 = 0, OB1 = OW1 + NH * KF, OW2 = OB1 + NH, OB2 = OW2 + NH * NH, OW3 = OB2 + NH, OB3 = OW3 + NH;
+constexpr int NPARAM = OB3 + 1;
+// BF16 copy layout
+constexpr int BW1 = 0, BW2 = BW1 + NH * KF, BW3 = BW2 + NH * NH, NB16 = BW3 + NH;
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ uint16_t f2bf(float f)
+{
+    uint32_t u = __float_as_uint(f);
+    u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest even
+    return (uint16_t)(u >> 16);
+}
+
+__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+// 12 standardised features of quartet (pb | pk) into f[0..11]
+__device__ __forceinline__ void quartet_features(const float *__restrict__ pf, int pb, int pk, float *f)
+{
+    const float *a = pf + 8 * (size_t)pb, *c = pf + 8 * (size_t)pk;
+    float dx = a[4] - c[4], dy = a[5] - c[5], dz = a[6] - c[6];
+    float r2 = dx * dx + dy * dy + dz * dz;
+    float za = a[7], zc = c[7];
+    float rho = za * zc / (za + zc);
+    f[0] = a[0];
+    f[1] = c[0];
+    f[2] = a[1];
+    f[3] = c[1];
+    f[4] = a[2];
+    f[5] = c[2];
+    f[6] = a[3];
+    f[7] = c[3];
+    f[8] = sqrtf(r2);
+    f[9] = log10f(rho);
+    f[10] = log10f(1.0f + rho * r2);
+    f[11] = a[0] + c[0];
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// features for training data: X[n][16] standardised, zero padded
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) intc_mlp_features_k(long long n, const float *__restrict__ pf,
+                                                           const int *__restrict__ qp, const float *__restrict__ norm,
+                                                           float *__restrict__ X)
+{
+    long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    float f[NF];
+    quartet_features(pf, qp[2 * q], qp[2 * q + 1], f);
+    for (int k = 0; k < NF; ++k) X[KF * q + k] = (f[k] - norm[k]) * norm[NF + k];
+    for (int k = NF; k < KF; ++k) X[KF * q + k] = 0.0f;
+}
+
+// ---------------------------------------------------------------------------
+// inference + routing on MFMA.  One wave per tile of 16 quartets.
+// v_mfma_f32_16x16x16_bf16 operands (lane l, slot j = 0..3):
+//   A[i][k]: i = l % 16, k = 4 (l / 16) + j
+//   B[k][n]: n = l % 16, k = 4 (l / 16) + j
+//   D[i][n]: n = l % 16, i = 4 (l / 16) + j
+// route: 2 if pred + m_skip < log_skip, else 1 if pred + m_32 < log_32 and
+// log10 Q_bra + log10 Q_ket < log_qq32, else 0.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) intc_mlp_route_k(long long n, const float *__restrict__ pf,
+                                                        const int *__restrict__ qp, const float *__restrict__ norm,
+                                                        const uint16_t *__restrict__ wb,
+                                                        const float *__restrict__ master, float m_skip,
+                                                        float log_skip, float m_32, float log_32, float log_qq32,
+                                                        float *__restrict__ pred, signed char *__restrict__ route)
+{
+    __shared__ uint16_t sW[NB16];
+    __shared__ float sB[2 * NH + NH + 1];
+    __shared__ float sH[4][16][NH + 1];
+    for (int i = threadIdx.x; i < NB16; i += blockDim.x) sW[i] = wb[i];
+    for (int i = threadIdx.x; i < NH; i += blockDim.x) {
+        sB[i] = master[OB1 + i];
+        sB[NH + i] = master[OB2 + i];
+        sB[2 * NH + i] = master[OW3 + i];
+    }
+    if (threadIdx.x == 0) sB[3 * NH] = master[OB3];
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int r16 = lane & 15, g4 = lane >> 4;
+    const long long ntile = (n + 15) / 16;
+    const long long stride = (long long)gridDim.x * (blockDim.x >> 6);
+    for (long long t = (long long)blockIdx.x * (blockDim.x >> 6) + wave; t < ntile; t += stride) {
+        // layer-1 A operand: row r16, features 4*g4 .. 4*g4+3
+        long long q = t * 16 + r16;
+        long long qc = q < n ? q : n - 1;
+        float f[NF];
+        quartet_features(pf, qp[2 * qc], qp[2 * qc + 1], f);
+        bf16x4 a1;
+        for (int j = 0; j < 4; ++j) {
+            int k = 4 * g4 + j;
+            float v = k < NF ? (f[k] - norm[k]) * norm[NF + k] : 0.0f;
+            a1[j] = (short)f2bf(v);
+        }
+        const float logqq = f[11];
+        // layer 1
+        for (int nt = 0; nt < 4; ++nt) {
+            bf16x4 b;
+            for (int j = 0; j < 4; ++j) b[j] = (short)sW[BW1 + (nt * 16 + r16) * KF + 4 * g4 + j];
+            f32x4 d = {0.f, 0.f, 0.f, 0.f};
+            d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a1, b, d, 0, 0, 0);
+            for (int j = 0; j < 4; ++j) {
+                float h = d[j] + sB[nt * 16 + r16];
+                sH[wave][4 * g4 + j][nt * 16 + r16] = h > 0.f ? h : 0.f;
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        // layer 2: K = 64 in 4 steps
+        f32x4 acc[4];
+        for (int nt = 0; nt < 4; ++nt) acc[nt] = f32x4{0.f, 0.f, 0.f, 0.f};
+        for (int ks = 0; ks < 4; ++ks) {
+            bf16x4 a;
+            for (int j = 0; j < 4; ++j) a[j] = (short)f2bf(sH[wave][r16][ks * 16 + 4 * g4 + j]);
+            for (int nt = 0; nt < 4; ++nt) {
+                bf16x4 b;
+                for (int j = 0; j < 4; ++j) b[j] = (short)sW[BW2 + (nt * 16 + r16) * NH + ks * 16 + 4 * g4 + j];
+                acc[nt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[nt], 0, 0, 0);
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        // layer 3: rows 4*g4 + j, columns r16 + 16 nt, reduced over r16
+        float part[4];
+        for (int j = 0; j < 4; ++j) {
+            float s = 0.f;
+            for (int nt = 0; nt < 4; ++nt) {
+                int col = nt * 16 + r16;
+                float h = acc[nt][j] + sB[NH + col];
+                h = h > 0.f ? h : 0.f;
+                s += bf2f(f2bf(h)) * bf2f(sW[BW3 + col]);
+            }
+            part[j] = s;
+        }
+        for (int off = 1; off < 16; off <<= 1)
+            for (int j = 0; j < 4; ++j) part[j] += __shfl_xor(part[j], off);
+        // lane with r16 == j writes row 4*g4 + j; its own quartet features are
+        // for row r16, so fetch log Q product of that row from lane 4*g4 + j
+        for (int j = 0; j < 4; ++j) {
+            int row = 4 * g4 + j;
+            float lq = __shfl(logqq, row);
+            if (r16 == j) {
+                long long qq = t * 16 + row;
+                if (qq < n) {
+                    float y = part[j] + sB[3 * NH];
+                    if (pred) pred[qq] = y;
+                    if (route) {
+                        signed char r = 0;
+                        if (y + m_skip < log_skip) r = 2;
+                        else if (y + m_32 < log_32 && lq < log_qq32) r = 1;
+                        route[qq] = r;
+                    }
+                }
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// training: one block of 64 threads handles 32 samples of the minibatch
+// ---------------------------------------------------------------------------
+constexpr int TS = 32;
+
+__global__ void __launch_bounds__(64) intc_mlp_train_k(int nb, const float *__restrict__ X,
+                                                       const float *__restrict__ T, const int *__restrict__ idx,
+                                                       const uint16_t *__restrict__ wb,
+                                                       const float *__restrict__ master, float *__restrict__ partial,
+                                                       float *__restrict__ loss)
+{
+    __shared__ float sX[TS][KF];
+    __shared__ float sH1[TS][NH + 1];
+    __shared__ float sH2[TS][NH + 1];
+    __shared__ float sD2[TS][NH + 1];
+    __shared__ float sD1[TS][NH + 1];
+    __shared__ float sDy[TS];
+    __shared__ float sW2[NH][NH + 1];
+    const int tid = threadIdx.x;
+    const int s0 = blockIdx.x * TS;
+    for (int i = tid; i < NH * NH; i += 64) sW2[i / NH][i % NH] = bf2f(wb[BW2 + i]);
+    for (int i = tid; i < TS * KF; i += 64) {
+        int s = i / KF, k = i % KF;
+        sX[s][k] = (s0 + s < nb) ? X[(size_t)KF * idx[s0 + s] + k] : 0.0f;
+    }
+    __syncthreads();
+    if (tid < TS) {
+        const int s = tid;
+        const bool valid = s0 + s < nb;
+        // forward with the BF16 weights
+        for (int h = 0; h < NH; ++h) {
+            float z = master[OB1 + h];
+            for (int k = 0; k < KF; ++k) z += bf2f(wb[BW1 + h * KF + k]) * bf2f(f2bf(sX[s][k]));
+            sH1[s][h] = z > 0.f ? z : 0.f;
+        }
+        float y = master[OB3];
+        for (int h = 0; h < NH; ++h) {
+            float z = master[OB2 + h];
+            for (int k = 0; k < NH; ++k) z += sW2[h][k] * bf2f(f2bf(sH1[s][k]));
+            z = z > 0.f ? z : 0.f;
+            sH2[s][h] = z;
+            y += bf2f(f2bf(z)) * bf2f(wb[BW3 + h]);
+        }
+        const float t = valid ? T[idx[s0 + s]] : 0.0f;
+        const float r = valid ? (y - t) : 0.0f;
+        sDy[s] = 2.0f * r / (float)nb;
+        if (valid && loss) atomicAdd(loss, r * r / (float)nb);
+        // backward through layer 3 and the second ReLU
+        for (int h = 0; h < NH; ++h) sD2[s][h] = sH2[s][h] > 0.f ? sDy[s] * bf2f(wb[BW3 + h]) : 0.0f;
+        // through W2 and the first ReLU
+        for (int k = 0; k < NH; ++k) {
+            float g = 0.f;
+            for (int h = 0; h < NH; ++h) g += sD2[s][h] * sW2[h][k];
+            sD1[s][k] = sH1[s][k] > 0.f ? g : 0.0f;
+        }
+    }
+    __syncthreads();
+    float *pg = partial + (size_t)blockIdx.x * (NPARAM + 1);
+    // dW2[h][k] = sum_s d2[s][h] * bf16(h1[s][k]); db2[h] = sum_s d2[s][h]
+    for (int i = tid; i < NH * NH; i += 64) {
+        int h = i / NH, k = i % NH;
+        float g = 0.f;
+        for (int s = 0; s < TS; ++s) g += sD2[s][h] * bf2f(f2bf(sH1[s][k]));
+        pg[OW2 + i] = g;
+    }
+    for (int i = tid; i < NH * KF; i += 64) {
+        int h = i / KF, k = i % KF;
+        float g = 0.f;
+        for (int s = 0; s < TS; ++s) g += sD1[s][h] * bf2f(f2bf(sX[s][k]));
+        pg[OW1 + i] = g;
+    }
+    {
+        const int h = tid;
+        float g1 = 0.f, g2 = 0.f, g3 = 0.f;
+        for (int s = 0; s < TS; ++s) {
+            g1 += sD1[s][h];
+            g2 += sD2[s][h];
+            g3 += sDy[s] * bf2f(f2bf(sH2[s][h]));
+        }
+        pg[OB1 + h] = g1;
+        pg[OB2 + h] = g2;
+        pg[OW3 + h] = g3;
+        if (h == 0) {
+            float g = 0.f;
+            for (int s = 0; s < TS; ++s) g += sDy[s];
+            pg[OB3] = g;
+        }
+    }
+}
+
+// sum of per-block partial gradients + Adam; refreshes the BF16 copies
+__global__ void __launch_bounds__(256) intc_mlp_adam_k(int nblk, const float *__restrict__ partial,
+                                                       float *__restrict__ master, float *__restrict__ m1,
+                                                       float *__restrict__ m2, uint16_t *__restrict__ wb, float lr,
+                                                       float b1, float b2, float eps, float c1, float c2,
+                                                       float *__restrict__ grad_out)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= NPARAM) return;
+    float g = 0.f;
+    for (int b = 0; b < nblk; ++b) g += partial[(size_t)b * (NPARAM + 1) + i];
+    // padding columns of W1 stay zero
+    if (i < OB1 && (i % KF) >= NF) g = 0.f;
+    if (grad_out) grad_out[i] = g;
+    if (lr > 0.f) {
+        float a = b1 * m1[i] + (1.f - b1) * g;
+        float v = b2 * m2[i] + (1.f - b2) * g * g;
+        m1[i] = a;
+        m2[i] = v;
+        master[i] -= lr * (a / c1) / (sqrtf(v / c2) + eps);
+    }
+    float w = master[i];
+    if (i < OB1) wb[BW1 + i] = f2bf(w);
+    else if (i >= OW2 && i < OB2) wb[BW2 + (i - OW2)] = f2bf(w);
+    else if (i >= OW3 && i < OB3) wb[BW3 + (i - OW3)] = f2bf(w);
+}
+
+// ---------------------------------------------------------------------------
+// C-ABI
+// ---------------------------------------------------------------------------
+INTC_API int intc_mlp_num_params(void) { return NPARAM; }
+INTC_API int intc_mlp_num_bf16(void) { return NB16; }
+INTC_API int intc_mlp_train_block(void) { return TS; }
+
+INTC_API int intc_mlp_features(long long n, const float *pair_feat, const int *quartet_pairs, const float *norm,
+                               float *X, void *stream)
+{
+    if (n <= 0) return 0;
+    hipLaunchKernelGGL(intc_mlp_features_k, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, n,
+                       pair_feat, quartet_pairs, norm, X);
+    return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+INTC_API int intc_mlp_route(long long n, const float *pair_feat, const int *quartet_pairs, const float *norm,
+                            const uint16_t *wb, const float *master, float m_skip, float log_skip, float m_32,
+                            float log_32, float log_qq32, float *pred, signed char *route, void *stream)
+{
+    if (n <= 0) return 0;
+    long long tiles = (n + 15) / 16;
+    int grid = (int)std::min<long long>((tiles + 3) / 4, 256 * 16);
+    hipLaunchKernelGGL(intc_mlp_route_k, dim3(grid), dim3(256), 0, (hipStream_t)stream, n, pair_feat,
+                       quartet_pairs, norm, wb, master, m_skip, log_skip, m_32, log_32, log_qq32, pred, route);
+    return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+// one minibatch of nb samples (indices idx into X/T); partial: nblk*(NPARAM+1)
+// floats with nblk = ceil(nb / train_block); loss accumulates the batch MSE
+INTC_API int intc_mlp_train_step(int nb, const float *X, const float *T, const int *idx, const uint16_t *wb,
+                                 const float *master, float *partial, float *loss, void *stream)
+{
+    if (nb <= 0) return 0;
+    int nblk = (nb + TS - 1) / TS;
+    hipLaunchKernelGGL(intc_mlp_train_k, dim3(nblk), dim3(64), 0, (hipStream_t)stream, nb, X, T, idx, wb, master,
+                       partial, loss);
+    return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+INTC_API int intc_mlp_adam(int nblk, const float *partial, float *master, float *m1, float *m2, uint16_t *wb,
+                           float lr, float beta1, float beta2, float eps, int step, float *grad_out, void *stream)
+{
+    float c1 = 1.f - powf(beta1, (float)step), c2 = 1.f - powf(beta2, (float)step);
+    if (lr <= 0.f) c1 = c2 = 1.f;
+    hipLaunchKernelGGL(intc_mlp_adam_k, dim3((NPARAM + 255) / 256), dim3(256), 0, (hipStream_t)stream, nblk, partial,
+                       master, m1, m2, wb, lr, beta1, beta2, eps, c1, c2, grad_out);
+    return hipGetLastError() == hipSuccess ? 0 : -3;
+}

