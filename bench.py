@@ -50,24 +50,167 @@ def cpu_baseline(batch, n_sample, seed=7):
             "quartets_per_s": float(sub.n_quartet / dt)}
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--quartets", type=int, default=1_000_000)
-    ap.add_argument("--cpu-sample", type=int, default=150_000)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-tune", action="store_true")
-    ap.add_argument("--streams", type=int, default=4)
-    ap.add_argument("--tune-out", default=None, help="also write the tuning table here")
-    ap.add_argument("--check", type=int, default=3000, help="quartets checked against the oracle")
-    args = ap.parse_args()
-    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU only: quartets shard across GPUs without exchange "
-              "(see the sharding design); run with --gpus 1", file=sys.stderr)
-        sys.exit(2)
+def _timed(torch, dev, stream, fn, n):
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize(dev)
+    e0.record(stream)
+    for _ in range(n):
+        fn()
+    e1.record(stream)
+    torch.cuda.synchronize(dev)
+    return e0.elapsed_time(e1) / n
 
+
+def _per_bin(torch, plan, out, flops_q, canon, stream, reps=3, prec=0):
+    """Isolated per-bin kernel times (events on the launch stream)."""
+    dev = plan.device
+    res = []
+    for b in plan.bins(prec):
+        ms = _timed(torch, dev, stream, lambda: plan.run_bin(b["bin"], out, prec), reps)
+        sel = np.all(canon == np.array(b["cls"]), axis=1)
+        res.append(dict(b, ms=ms, flops=float(flops_q[sel].sum())))
+    return res
+
+
+def run_config3(args):
+    """Molecule-like screened workload: plain FP64 vs learned screening +
+    FP32/FP64 routing over every Schwarz-screened quartet of the molecule."""
+    import torch
+    from intception_amd import autotune, basis, flops_model, lib, runtime, screening
+    from intception_amd.codegen import canonical_np
+    lib.check(lib.load().intc_init())
+    t0 = time.perf_counter()
+    router, rstats = screening.build_router(n_train=args.train, n_atoms=args.atoms, epochs=15, n_val=1_000_000,
+                                            log=log)
+    t_router = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    mol, pf = screening.molecule_setup(args.atoms, 202603)
+    pp = mol.screened_quartets(1e-12)
+    batch = mol.pair_batch(pp)
+    t_setup = time.perf_counter() - t0
+    log("config3 molecule: %d atoms, %d shells, %d pairs, %d screened quartets, %d integrals (%.1fs)"
+        % (args.atoms, mol.n_shell, len(mol.pairs), batch.n_quartet, batch.out_size, t_setup))
+    t0 = time.perf_counter()
+    plan = runtime.EriPlan(batch)
+    t_plan = time.perf_counter() - t0
+    log("plan: %d bins (%.1fs)" % (len(plan.bins()), t_plan))
+    dev = plan.device
+    stream = torch.cuda.current_stream(dev)
+    out64 = plan.alloc_out()
+    outml = plan.alloc_out()
+    d_pf = torch.from_numpy(pf).to(dev)
+    d_qp = torch.from_numpy(np.ascontiguousarray(pp, dtype=np.int32)).to(dev)
+    route = torch.empty(batch.n_quartet, dtype=torch.int8, device=dev)
+    plan.run(out64)      # builds the primitive-pair arrays before any per-bin launch
+    n_cached = autotune.apply_cache(plan)
+    if not args.no_tune:
+        autotune.tune(plan, out64, reps=2, log=None, save_path=args.tune_out, precs=(0, 1))
+    log("tuning: %d cached, tuned=%s" % (n_cached, not args.no_tune))
+    # plain FP64: one graph, every screened quartet
+    plan.capture(out64, n_streams=args.streams)
+    for _ in range(args.warmup):
+        plan.replay()
+    ms64 = _timed(torch, dev, stream, plan.replay, args.steps)
+    log("plain FP64: %.3f ms/step" % ms64)
+    sizes = torch.from_numpy(batch.sizes()).to(dev)
+
+    def run_learned(use_fp32):
+        """router kernel (features + MFMA network + route codes) then the
+        routed graph (pair builder, route compaction, FP64 [and FP32] queues)"""
+        plan.set_route_fp32(use_fp32)
+        plan.capture(outml, n_streams=args.streams, route=route)
+
+        def step():
+            router.route(d_pf, d_qp, route)
+            plan.replay()
+
+        for _ in range(args.warmup):
+            step()
+        ms = _timed(torch, dev, stream, step, args.steps)
+        codes = torch.bincount(route.to(torch.int64), minlength=3).cpu().numpy()
+        if not use_fp32:
+            codes = np.array([codes[0] + codes[1], 0, codes[2]])
+        per_code = torch.repeat_interleave(route, sizes)
+        if not use_fp32:
+            per_code = torch.where(per_code == 1, torch.zeros_like(per_code), per_code)
+        errs = [0.0, 0.0, 0.0]
+        chunk = 1 << 28
+        for s0 in range(0, batch.out_size, chunk):
+            d = (outml[s0:s0 + chunk] - out64[s0:s0 + chunk]).abs()
+            pc = per_code[s0:s0 + chunk]
+            for c in range(3):
+                if codes[c]:
+                    errs[c] = max(errs[c], float(torch.where(pc == c, d, torch.zeros_like(d)).max().item()))
+        del per_code
+        err = max(errs)
+        log("learned (fp32 %s): %.3f ms/step; fp64/fp32/skip = %s; max abs err %.3e (by route %s)"
+            % (use_fp32, ms, codes.tolist(), err, ["%.2e" % e for e in errs]))
+        return {"ms_per_step": float(ms), "quartets_per_s": float(batch.n_quartet / (ms * 1e-3)),
+                "integrals_per_s": float(batch.out_size / (ms * 1e-3)),
+                "route_fraction_fp64_fp32_skip": (codes / batch.n_quartet).tolist(),
+                "max_abs_err_vs_fp64": err, "max_abs_err_by_route_fp64_fp32_skip": errs,
+                "meets_1e-10": bool(err <= 1e-10)}
+
+    ms_route = _timed(torch, dev, stream, lambda: router.route(d_pf, d_qp, route), args.steps)
+    log("router kernel: %.3f ms" % ms_route)
+    learned_skip = run_learned(False)
+    learned_mixed = run_learned(True)
+    ok = [v for v in (learned_mixed, learned_skip) if v["meets_1e-10"]]
+    best = min(ok, key=lambda v: v["ms_per_step"]) if ok else learned_skip
+    msml = best["ms_per_step"]
+    flops_q = flops_model.batch_flops(batch)
+    canon = canonical_np(batch.classes())
+    per_bin = _per_bin(torch, plan, out64, flops_q, canon, stream)
+    dom = max(per_bin, key=lambda r: r["ms"])
+    achieved = dom["flops"] / (dom["ms"] * 1e-3) / 1e12
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(batch, args.cpu_sample)
+        log("cpu baseline:", cpu)
+    n_int = batch.out_size
+    res = {
+        "metric": "contracted ERIs/s on 1 MI355X: FP64 vs mixed+learned at max err <=1e-10",
+        "value": float(n_int / (msml * 1e-3)),
+        "unit": "integrals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": float(msml),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64+f32 (learned routing)",
+        "data": "synthetic",
+        "config": {"workload": "config3: %d-atom molecule (random sequential packing, r = %.1f bohr, spacing "
+                               ">= 1.8), shells s,s,p,p,d K=4,1,3,1,1 even-tempered; all %d unique shell quartets "
+                               "passing Schwarz 1e-12; spherical" % (args.atoms, mol.radius, batch.n_quartet),
+                   "quartets": int(batch.n_quartet), "integrals": int(n_int), "seed": 202603,
+                   "router_training": "%d quartets from molecules seeded 202611/202612, calibrated on 202619"
+                                      % rstats["train_samples"]},
+        "plain_fp64": {"ms_per_step": float(ms64), "integrals_per_s": float(n_int / (ms64 * 1e-3)),
+                       "quartets_per_s": float(batch.n_quartet / (ms64 * 1e-3))},
+        "learned_skip_fp64": learned_skip,
+        "learned_skip_and_fp32": learned_mixed,
+        "value_from": "learned_skip_and_fp32" if best is learned_mixed else "learned_skip_fp64",
+        "router_ms": float(ms_route),
+        "speedup_vs_plain_fp64": float(ms64 / msml),
+        "router": {k: v for k, v in rstats.items() if k != "train_mse"},
+        "roofline": {"bound": "mfma", "achieved": float(achieved), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": float(achieved / FP64_PEAK_TFLOPS), "traffic": None,
+                     "kernel": "intc_eri_d%d%d%d%d_s" % tuple(dom["cls"]), "kernel_ms": dom["ms"],
+                     "kernel_flops": dom["flops"],
+                     "note": "dominant FP64 class kernel of the plain path; FP64 peak 78.6 TF/s (vector = "
+                             "matrix on MI355X); flops = reference-scheme count (intception_amd/flops_model.py)",
+                     "job_frac_plain_fp64": float(flops_q.sum() / (ms64 * 1e-3) / 1e12 / FP64_PEAK_TFLOPS)},
+        "cpu_baseline": cpu,
+        "setup_s": {"router_train_and_calibrate": t_router, "molecule_and_screening": t_setup, "plan": t_plan},
+        "bins": [{"cls": r["cls"], "nq": r["nq"], "ms": round(r["ms"], 4)} for r in per_bin],
+    }
+    plan.close()
+    return res
+
+
+def run_config1(args):
     import torch
     from intception_amd import basis, flops_model, lib, runtime
 
@@ -119,9 +262,8 @@ def main():
     # per-kernel durations (events on the launch stream) for the roofline
     bins = plan.bins()
     flops_q = flops_model.batch_flops(batch)
-    classes = batch.classes()
-    from intception_amd.codegen import canonical
-    canon = np.array([canonical(*c)[0] for c in map(tuple, classes)])
+    from intception_amd.codegen import canonical_np
+    canon = canonical_np(batch.classes())
     reps = max(3, args.steps)
     per_bin = []
     torch.cuda.synchronize(dev)
@@ -205,8 +347,32 @@ def main():
         "bins": [{"cls": r["cls"], "nq": r["nq"], "T": r["team_groups"], "ms": round(r["ms"], 4),
                   "tflops": round(r["flops"] / (r["ms"] * 1e-3) / 1e12, 3)} for r in per_bin],
     }
-    print(json.dumps(res), flush=True)
     plan.close()
+    return res
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", choices=["config3", "config1"], default="config3")
+    ap.add_argument("--quartets", type=int, default=1_000_000, help="config1 size")
+    ap.add_argument("--atoms", type=int, default=200, help="config3 molecule size")
+    ap.add_argument("--train", type=int, default=2_000_000, help="config3 training quartets")
+    ap.add_argument("--cpu-sample", type=int, default=150_000)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--streams", type=int, default=4)
+    ap.add_argument("--tune-out", default=None, help="also write the tuning table here")
+    ap.add_argument("--check", type=int, default=3000, help="quartets checked against the oracle")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: quartets shard across GPUs without exchange "
+              "(see the sharding design); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    res = run_config3(args) if args.workload == "config3" else run_config1(args)
+    print(json.dumps(res), flush=True)
 
 
 if __name__ == "__main__":

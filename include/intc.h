@@ -99,6 +99,7 @@ int intc_plan_set_bin_tuning(intc_plan *plan, int bin, int prec, int team_groups
  * column-major (NULL for Cartesian plans).  stream: hipStream_t or NULL. */
 int intc_plan_build_pairs(intc_plan *plan, const double *exps, const double *coefs, const double *shell_cen,
                           void *stream);
+/* One class bin; the pair data must have been built (build_pairs or execute). */
 int intc_plan_run_bin(intc_plan *plan, int bin, int prec, const double *sph, double *out, void *stream);
 /* All quartets in FP64. */
 int intc_plan_execute(intc_plan *plan, const double *exps, const double *coefs, const double *shell_cen,
@@ -115,6 +116,10 @@ int intc_plan_execute_routed(intc_plan *plan, const double *exps, const double *
 int intc_plan_graph_create(intc_plan *plan, const double *exps, const double *coefs, const double *shell_cen,
                            const double *sph, const signed char *route, double *out, int n_streams, void *stream);
 int intc_plan_graph_launch(intc_plan *plan, void *stream);
+/* Routed runs: 1 (default) launches FP64 and FP32-recurrence queues; 0 treats
+ * route code 1 as FP64 and launches the FP64 kernels only (no effect on
+ * graphs already captured). */
+int intc_plan_set_route_fp32(intc_plan *plan, int use_fp32);
 /* FP64/FP32 queue lengths per bin of the last routed execution (2 ints per bin). */
 int intc_plan_route_counts(intc_plan *plan, int *counts);
 int intc_plan_destroy(intc_plan *plan);

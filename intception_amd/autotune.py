@@ -56,13 +56,15 @@ def save_cache(cache, path=CACHE):
 
 def bin_mean_prims(plan):
     """Mean primitive-quartet count of each bin, host side."""
-    from .codegen import canonical
+    from .codegen import canonical_np
     b = plan.batch
     P = b.shell_nprim[b.quartets].astype(np.int64).prod(axis=1)
-    canon = np.array([canonical(*c)[0] for c in map(tuple, b.classes())])
+    canon = canonical_np(b.classes())
+    key = canon[:, 0] * 1000 + canon[:, 1] * 100 + canon[:, 2] * 10 + canon[:, 3]
     out = {}
     for r in plan.bins():
-        sel = np.all(canon == np.array(r["cls"]), axis=1)
+        c = r["cls"]
+        sel = key == c[0] * 1000 + c[1] * 100 + c[2] * 10 + c[3]
         out[r["bin"]] = float(P[sel].mean()) if sel.any() else 1.0
     return out
 
@@ -95,6 +97,7 @@ def tune(plan, out, reps=3, boys_mode=0, grid_modes=("auto", 4, 8), save=True, l
     means = bin_mean_prims(plan)
     cache = load_cache()
     table = []
+    plan.build_pairs()   # the class kernels read the primitive-pair arrays
     for prec in precs:
         for r in plan.bins(prec):
             best = None

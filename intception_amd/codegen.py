@@ -31,6 +31,8 @@ Canonical classes have la >= lb, lc >= ld and the smaller pair in the bra
 import itertools
 import os
 
+import numpy as np
+
 from .basis import ncc, ncc0, nsph, cart_components
 
 
@@ -79,6 +81,15 @@ def canonical(la, lb, lc, ld):
     cls = (bra[0][0], bra[1][0], ket[0][0], ket[1][0])
     perm = (bra[0][1], bra[1][1], ket[0][1], ket[1][1])
     return cls, perm
+
+
+def canonical_np(ls):
+    """Vectorised canonical class for an (n, 4) array of angular momenta."""
+    ls = np.asarray(ls)
+    a, b = np.maximum(ls[:, 0], ls[:, 1]), np.minimum(ls[:, 0], ls[:, 1])
+    c, d = np.maximum(ls[:, 2], ls[:, 3]), np.minimum(ls[:, 2], ls[:, 3])
+    sw = (a + b > c + d) | ((a + b == c + d) & (a > c))
+    return np.stack([np.where(sw, c, a), np.where(sw, d, b), np.where(sw, a, c), np.where(sw, b, d)], axis=1)
 
 
 def canonical_classes(lmax):

@@ -74,6 +74,9 @@ __device__ __forceinline__ float intc_shflf(float v, int src4)
 template <int M, int BM>
 __device__ __forceinline__ void intc_boys(double T, double *F, const IntcEriArgs &a)
 {
+    // T = rho |PQ|^2 >= 0; the clamp (fmax maps NaN to 0) keeps every table
+    // index below in range whatever the input
+    T = fmax(T, 0.0);
     if (BM == 0) {
         if (T < (double)INTC_BOYS_NINT) {
             int k = (int)T;

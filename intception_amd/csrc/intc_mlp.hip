@@ -1,6 +1,7 @@
 // intc_mlp.hip -- the quartet-magnitude network: 12 -> 64 -> 64 -> 1, ReLU.
 //
-// Predicts log10(max |integral|) of a contracted quartet from pair features,
+// Predicts log10(max |integral| / (Q_bra Q_ket)) of a contracted quartet (the
+// decay below the Schwarz bound, clamped at 0) from pair features,
 // to tighten Schwarz screening (skip) and to route quartets between the FP64
 // and FP32-recurrence kernels.  Weights are stored as BF16 for inference
 // (FP32 master copies for training); every product accumulates in FP32.
@@ -17,7 +18,8 @@
 // log10 zeta_min, |AB|, la + lb, P_x, P_y, P_z (centre of the most diffuse
 // primitive pair), zeta_min.  Quartet features (12, standardised with the
 // model's mean/scale): log10 Q_bra, log10 Q_ket, log10 zeta_bra, log10 zeta_ket,
-// |AB|, |CD|, L_bra, L_ket, R_PQ, log10 rho, log10(1 + rho R_PQ^2),
+// |AB|, |CD|, L_bra, L_ket, R_PQ, log10 rho, s-type Coulomb decay
+// log10[F0(rho R_PQ^2) (4 zeta_bra zeta_ket)^(1/4) / sqrt(zeta_bra + zeta_ket)],
 // log10 Q_bra + log10 Q_ket.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -65,7 +67,11 @@ __device__ __forceinline__ void quartet_features(const float *__restrict__ pf, i
     f[7] = c[3];
     f[8] = sqrtf(r2);
     f[9] = log10f(rho);
-    f[10] = log10f(1.0f + rho * r2);
+    // s-type Coulomb decay below the Schwarz bound for the diffuse pairs:
+    // (ab|cd) / sqrt((ab|ab)(cd|cd)) = F0(T) (2 za)^(1/4) (2 zc)^(1/4) / sqrt(za + zc)
+    const float T = rho * r2;
+    const float f0 = T < 1e-6f ? 1.0f : 0.886226925f * erff(sqrtf(T)) / sqrtf(T);
+    f[10] = log10f(f0) + 0.25f * log10f(4.0f * za * zc) - 0.5f * log10f(za + zc);
     f[11] = a[0] + c[0];
 }
 
@@ -115,6 +121,14 @@ __global__ void __launch_bounds__(256) intc_mlp_route_k(long long n, const float
     __syncthreads();
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int r16 = lane & 15, g4 = lane >> 4;
+    // B operands of layers 1 and 2 stay in registers for every tile
+    bf16x4 b1w[4], b2w[4][4];
+    for (int nt = 0; nt < 4; ++nt) {
+        for (int j = 0; j < 4; ++j) b1w[nt][j] = (short)sW[BW1 + (nt * 16 + r16) * KF + 4 * g4 + j];
+        for (int ks = 0; ks < 4; ++ks)
+            for (int j = 0; j < 4; ++j)
+                b2w[ks][nt][j] = (short)sW[BW2 + (nt * 16 + r16) * NH + ks * 16 + 4 * g4 + j];
+    }
     const long long ntile = (n + 15) / 16;
     const long long stride = (long long)gridDim.x * (blockDim.x >> 6);
     for (long long t = (long long)blockIdx.x * (blockDim.x >> 6) + wave; t < ntile; t += stride) {
@@ -132,10 +146,8 @@ __global__ void __launch_bounds__(256) intc_mlp_route_k(long long n, const float
         const float logqq = f[11];
         // layer 1
         for (int nt = 0; nt < 4; ++nt) {
-            bf16x4 b;
-            for (int j = 0; j < 4; ++j) b[j] = (short)sW[BW1 + (nt * 16 + r16) * KF + 4 * g4 + j];
             f32x4 d = {0.f, 0.f, 0.f, 0.f};
-            d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a1, b, d, 0, 0, 0);
+            d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a1, b1w[nt], d, 0, 0, 0);
             for (int j = 0; j < 4; ++j) {
                 float h = d[j] + sB[nt * 16 + r16];
                 sH[wave][4 * g4 + j][nt * 16 + r16] = h > 0.f ? h : 0.f;
@@ -149,11 +161,8 @@ __global__ void __launch_bounds__(256) intc_mlp_route_k(long long n, const float
         for (int ks = 0; ks < 4; ++ks) {
             bf16x4 a;
             for (int j = 0; j < 4; ++j) a[j] = (short)f2bf(sH[wave][r16][ks * 16 + 4 * g4 + j]);
-            for (int nt = 0; nt < 4; ++nt) {
-                bf16x4 b;
-                for (int j = 0; j < 4; ++j) b[j] = (short)sW[BW2 + (nt * 16 + r16) * NH + ks * 16 + 4 * g4 + j];
-                acc[nt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[nt], 0, 0, 0);
-            }
+            for (int nt = 0; nt < 4; ++nt)
+                acc[nt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b2w[ks][nt], acc[nt], 0, 0, 0);
         }
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         __builtin_amdgcn_wave_barrier();
@@ -179,7 +188,8 @@ __global__ void __launch_bounds__(256) intc_mlp_route_k(long long n, const float
             if (r16 == j) {
                 long long qq = t * 16 + row;
                 if (qq < n) {
-                    float y = part[j] + sB[3 * NH];
+                    // the network predicts log10(max|integral| / (Q_bra Q_ket)) <= 0
+                    float y = lq + fminf(part[j] + sB[3 * NH], 0.0f);
                     if (pred) pred[qq] = y;
                     if (route) {
                         signed char r = 0;
@@ -204,7 +214,7 @@ __global__ void __launch_bounds__(64) intc_mlp_train_k(int nb, const float *__re
                                                        const float *__restrict__ T, const int *__restrict__ idx,
                                                        const uint16_t *__restrict__ wb,
                                                        const float *__restrict__ master, float *__restrict__ partial,
-                                                       float *__restrict__ loss)
+                                                       float *__restrict__ loss, float under_weight)
 {
     __shared__ float sX[TS][KF];
     __shared__ float sH1[TS][NH + 1];
@@ -240,8 +250,10 @@ __global__ void __launch_bounds__(64) intc_mlp_train_k(int nb, const float *__re
         }
         const float t = valid ? T[idx[s0 + s]] : 0.0f;
         const float r = valid ? (y - t) : 0.0f;
-        sDy[s] = 2.0f * r / (float)nb;
-        if (valid && loss) atomicAdd(loss, r * r / (float)nb);
+        // weighted squared error: under-predictions (y < t) weigh under_weight
+        const float w = r < 0.0f ? under_weight : 1.0f;
+        sDy[s] = 2.0f * w * r / (float)nb;
+        if (valid && loss) atomicAdd(loss, w * r * r / (float)nb);
         // backward through layer 3 and the second ReLU
         for (int h = 0; h < NH; ++h) sD2[s][h] = sH2[s][h] > 0.f ? sDy[s] * bf2f(wb[BW3 + h]) : 0.0f;
         // through W2 and the first ReLU
@@ -341,14 +353,15 @@ INTC_API int intc_mlp_route(long long n, const float *pair_feat, const int *quar
 }
 
 // one minibatch of nb samples (indices idx into X/T); partial: nblk*(NPARAM+1)
-// floats with nblk = ceil(nb / train_block); loss accumulates the batch MSE
+// floats with nblk = ceil(nb / train_block); loss accumulates the batch's
+// weighted MSE (under-predictions weigh under_weight)
 INTC_API int intc_mlp_train_step(int nb, const float *X, const float *T, const int *idx, const uint16_t *wb,
-                                 const float *master, float *partial, float *loss, void *stream)
+                                 const float *master, float *partial, float *loss, float under_weight, void *stream)
 {
     if (nb <= 0) return 0;
     int nblk = (nb + TS - 1) / TS;
     hipLaunchKernelGGL(intc_mlp_train_k, dim3(nblk), dim3(64), 0, (hipStream_t)stream, nb, X, T, idx, wb, master,
-                       partial, loss);
+                       partial, loss, under_weight);
     return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 

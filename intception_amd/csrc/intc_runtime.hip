@@ -175,6 +175,8 @@ struct intc_plan {
     // device pair data (filled by execute)
     double *pz = nullptr, *po2z = nullptr, *ppx = nullptr, *ppy = nullptr, *ppz = nullptr, *pkap = nullptr;
     double *pairA = nullptr, *pairAB = nullptr;
+    bool pairs_built = false;   // primitive-pair arrays written at least once
+    bool route_fp32 = true;     // routed runs launch the FP32-recurrence queues
     // captured execution
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
@@ -245,7 +247,7 @@ __global__ void __launch_bounds__(256) intc_route_compact(
     long long nq, const signed char *__restrict__ route, const int *__restrict__ q_orig,
     const int *__restrict__ q_bin, const int *__restrict__ bin_q0, const long long *__restrict__ q_off,
     const int *__restrict__ q_size, int *__restrict__ counts, int *__restrict__ idx64, int *__restrict__ idx32,
-    double *__restrict__ out)
+    double *__restrict__ out, int use32)
 {
     long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const bool live = i < nq;
@@ -254,6 +256,7 @@ __global__ void __launch_bounds__(256) intc_route_compact(
         r = route[q_orig[i]];
         b = q_bin[i];
         if (r < 0 || r > 2) r = 0;
+        if (r == 1 && !use32) r = 0;
     }
     // a wave's quartets are consecutive in plan order and mostly share a bin;
     // lanes whose bin differs from lane 0's take the slow path
@@ -680,6 +683,7 @@ INTC_API int intc_plan_build_pairs(intc_plan *p, const double *exps, const doubl
                        p->pair_sb, p->pair_off, p->pair_nk, p->shell_off, shell_cen, exps, coefs, p->pz, p->po2z,
                        p->ppx, p->ppy, p->ppz, p->pkap, p->pairA, p->pairAB);
     HIPCHK(hipGetLastError());
+    p->pairs_built = true;
     return 0;
 }
 
@@ -688,6 +692,8 @@ INTC_API int intc_plan_run_bin(intc_plan *p, int i, int prec, const double *sph,
     if (i < 0 || i >= (int)p->bins.size()) return fail(-1, "bin index out of range");
     if (prec != 0 && prec != 1) return fail(-1, "prec must be 0 (FP64) or 1 (FP32 recurrences)");
     if (p->spherical && !sph) return fail(-1, "spherical plan needs the S matrices");
+    if (!p->pairs_built && p->n_pp > 0)
+        return fail(-1, "pair data not built: call intc_plan_build_pairs (or execute) first");
     IntcEriArgs a;
     int rc = fill_args(p, sph, out, &a);
     if (rc) return rc;
@@ -703,7 +709,8 @@ static int route_queues(intc_plan *p, const signed char *route, double *out, hip
     HIPCHK(hipGetLastError());
     int blocks = (int)((p->n_quartet + 255) / 256);
     hipLaunchKernelGGL(intc_route_compact, dim3(blocks), dim3(256), 0, st, p->n_quartet, route, p->q_orig,
-                       p->q_bin, p->bin_q0, p->q_off, p->q_size, p->counts, p->idx64, p->idx32, out);
+                       p->q_bin, p->bin_q0, p->q_off, p->q_size, p->counts, p->idx64, p->idx32, out,
+                       p->route_fp32 ? 1 : 0);
     HIPCHK(hipGetLastError());
     return 0;
 }
@@ -731,14 +738,22 @@ static int run_all(intc_plan *p, const double *sph, double *out, const signed ch
                (double)p->bins[y].nq * p->bins[y].cls[0]->NE * p->bins[y].cls[0]->NCD;
     });
     int k = 0;
+    const int nprec = route ? (p->route_fp32 ? 2 : 1) : 1;
     for (int bi : ord) {
         const IntcBin &b = p->bins[bi];
-        for (int prec = 0; prec < (route ? 2 : 1); ++prec) {
+        for (int prec = 0; prec < nprec; ++prec) {
             hipStream_t s2 = (k % n_streams) == 0 ? st : p->side[(k % n_streams) - 1];
             ++k;
             const int *qi = route ? (prec == 0 ? p->idx64 : p->idx32) + b.q0 : nullptr;
             const int *nd = route ? p->counts + 2 * bi + prec : nullptr;
-            rc = launch_bin(p, b, prec, a, qi, nd, s2);
+            if (route) {
+                // queue lengths are only known on the device: persistent grid
+                IntcBin br = b;
+                br.grid[prec] = std::min(b.grid[prec], 256 * 16);
+                rc = launch_bin(p, br, prec, a, qi, nd, s2);
+            } else {
+                rc = launch_bin(p, b, prec, a, qi, nd, s2);
+            }
             if (rc) return rc;
         }
     }
@@ -837,6 +852,14 @@ INTC_API int intc_plan_graph_launch(intc_plan *p, void *stream)
 {
     if (!p->graph_exec) return fail(-1, "no graph captured for this plan");
     HIPCHK(hipGraphLaunch(p->graph_exec, (hipStream_t)stream));
+    return 0;
+}
+
+// routed runs: 1 = FP64 and FP32-recurrence queues (default), 0 = route code
+// 1 is treated as FP64 and only the FP64 kernels are launched
+INTC_API int intc_plan_set_route_fp32(intc_plan *p, int use_fp32)
+{
+    p->route_fp32 = use_fp32 != 0;
     return 0;
 }
 

@@ -8,7 +8,9 @@ libintc's HIP kernels (csrc/intc_mlp.hip); torch provides device memory.
 * ``pair_features(mol)``: per shell pair log10 Q, log10 zeta_min, |AB|, L and
   the centre/exponent of the most diffuse primitive pair (8 floats).
 * ``training_set(...)``: quartets of separate molecules (their own seeds),
-  targets log10 max|integral| from the device FP64 path.
+  targets log10(max|integral| / (Q_bra Q_ket)) from the device FP64 path (the
+  decay below the Schwarz bound; the network's output is clamped at 0, so its
+  bound is never looser than Schwarz).
 * ``MagnitudeNet.fit`` trains on the device; ``calibrate`` sets the margins
   from held-out quartets so that skipped quartets stay below the error budget;
   ``route`` writes the per-quartet route codes on the device.
@@ -41,7 +43,7 @@ def _L():
         f, vp, ll, i = ctypes.c_float, ctypes.c_void_p, ctypes.c_longlong, ctypes.c_int
         L.intc_mlp_features.argtypes = [ll, vp, vp, vp, vp, vp]
         L.intc_mlp_route.argtypes = [ll, vp, vp, vp, vp, vp, f, f, f, f, f, vp, vp, vp]
-        L.intc_mlp_train_step.argtypes = [i, vp, vp, vp, vp, vp, vp, vp, vp]
+        L.intc_mlp_train_step.argtypes = [i, vp, vp, vp, vp, vp, vp, vp, f, vp]
         L.intc_mlp_adam.argtypes = [i, vp, vp, vp, vp, vp, f, f, f, f, i, vp, vp]
         for n in ("intc_mlp_features", "intc_mlp_route", "intc_mlp_train_step", "intc_mlp_adam",
                   "intc_mlp_num_params", "intc_mlp_num_bf16", "intc_mlp_train_block"):
@@ -81,8 +83,13 @@ def quartet_features_np(pf, qp):
     a, c = pf[qp[:, 0]].astype(np.float64), pf[qp[:, 1]].astype(np.float64)
     r2 = np.sum((a[:, 4:7] - c[:, 4:7]) ** 2, axis=1)
     rho = a[:, 7] * c[:, 7] / (a[:, 7] + c[:, 7])
+    T = rho * r2
+    sT = np.sqrt(np.maximum(T, 1e-300))
+    from scipy.special import erf
+    f0 = np.where(T < 1e-6, 1.0, 0.886226925452758 * erf(sT) / sT)
+    decay = np.log10(f0) + 0.25 * np.log10(4 * a[:, 7] * c[:, 7]) - 0.5 * np.log10(a[:, 7] + c[:, 7])
     return np.stack([a[:, 0], c[:, 0], a[:, 1], c[:, 1], a[:, 2], c[:, 2], a[:, 3], c[:, 3], np.sqrt(r2),
-                     np.log10(rho), np.log10(1 + rho * r2), a[:, 0] + c[:, 0]], axis=1)
+                     np.log10(rho), decay, a[:, 0] + c[:, 0]], axis=1)
 
 
 def block_log_max(batch, out):
@@ -112,7 +119,7 @@ def unpack(master):
     return W1, master[OB1:OW2], master[OW2:OB2].reshape(NH, NH), master[OB2:OW3], master[OW3:OB3], master[OB3]
 
 
-def reference_forward_backward(master, X, T, rounding=True):
+def reference_forward_backward(master, X, T, rounding=True, under_weight=1.0):
     """FP64 numpy restatement of intc_mlp_train_k + the Adam gradient sum:
     forward with BF16-rounded weights and activations, MSE loss, gradient
     w.r.t. every master parameter (straight-through for the rounding).
@@ -129,8 +136,9 @@ def reference_forward_backward(master, X, T, rounding=True):
     h2b = rnd(h2)
     y = h2b @ w3b + b3
     r = y - T
-    loss = np.mean(r * r)
-    dy = 2 * r / n
+    w = np.where(r < 0, under_weight, 1.0)
+    loss = np.mean(w * r * r)
+    dy = 2 * w * r / n
     d2 = (dy[:, None] * w3b[None, :]) * (h2 > 0)
     d1 = (d2 @ W2b) * (h1 > 0)
     g = np.zeros(NPARAM)
@@ -198,7 +206,7 @@ class MagnitudeNet:
         _lib.check(_L().intc_mlp_features(n, _p(d_pair_feat), _p(d_qp), _p(self.d_norm), _p(X), self._stream()))
         return X
 
-    def grad(self, X, T, idx):
+    def grad(self, X, T, idx, under_weight=1.0):
         """Loss and summed gradient of one minibatch (device kernels)."""
         t = self.torch
         nb = idx.shape[0]
@@ -206,14 +214,15 @@ class MagnitudeNet:
         part = t.empty(nblk * (NPARAM + 1), dtype=t.float32, device=self.device)
         loss = t.zeros(1, dtype=t.float32, device=self.device)
         _lib.check(_L().intc_mlp_train_step(nb, _p(X), _p(T), _p(idx), _p(self.wb), _p(self.master), _p(part),
-                                            _p(loss), self._stream()))
+                                            _p(loss), float(under_weight), self._stream()))
         g = t.empty(NPARAM, dtype=t.float32, device=self.device)
         _lib.check(_L().intc_mlp_adam(nblk, _p(part), _p(self.master), _p(self.m1), _p(self.m2), _p(self.wb), 0.0,
                                       0.9, 0.999, 1e-8, 1, _p(g), self._stream()))
         return loss, g
 
-    def fit(self, X, T, epochs=8, batch=4096, lr=2e-3, log=None, seed=0):
-        """Adam on the device over shuffled minibatches of (X, T)."""
+    def fit(self, X, T, epochs=8, batch=4096, lr=2e-3, log=None, seed=0, under_weight=1.0):
+        """Adam on the device over shuffled minibatches of (X, T); squared
+        errors of under-predictions weigh under_weight."""
         t = self.torch
         n = X.shape[0]
         gen = t.Generator(device=self.device)
@@ -233,7 +242,7 @@ class MagnitudeNet:
                 nblk = -(-nb // tb)
                 self.step += 1
                 _lib.check(_L().intc_mlp_train_step(nb, _p(X), _p(T), _p(idx), _p(self.wb), _p(self.master),
-                                                    _p(part), _p(loss), self._stream()))
+                                                    _p(part), _p(loss), float(under_weight), self._stream()))
                 _lib.check(_L().intc_mlp_adam(nblk, _p(part), _p(self.master), _p(self.m1), _p(self.m2),
                                               _p(self.wb), float(lr_ep), 0.9, 0.999, 1e-8, self.step, None,
                                               self._stream()))
@@ -246,6 +255,7 @@ class MagnitudeNet:
 
     def predict(self, d_pair_feat, d_qp, route=None, m_skip=0.0, log_skip=-99.0, m_32=0.0, log_32=-99.0,
                 log_qq32=-99.0):
+        """Predicted log10 max|integral| = log10(Q_bra Q_ket) + min(0, network)."""
         t = self.torch
         n = d_qp.shape[0]
         pred = t.empty(n, dtype=t.float32, device=self.device)

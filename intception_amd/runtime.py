@@ -118,6 +118,9 @@ class EriPlan:
                                                     _ptr(self.d_cen), sph, _ptr(route), _ptr(out), self._stream()))
         return out
 
+    def set_route_fp32(self, use_fp32):
+        _lib.check(self._L.intc_plan_set_route_fp32(self.handle, int(bool(use_fp32))))
+
     def route_counts(self):
         n = self._L.intc_plan_num_bins(self.handle)
         buf = (ctypes.c_int * (2 * max(n, 1)))()

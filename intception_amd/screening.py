@@ -54,12 +54,15 @@ def sample_targets(mol, n, seed, tau=1e-12, fp32=False):
 
 
 def training_set(n_samples, seeds, n_atoms=200, tau=1e-12, log=None):
+    """Features and targets log10(max|integral| / (Q_P Q_P')) (clamped to
+    [-25, 0]) for quartets of molecules with the given seeds."""
     feats, targets = [], []
     for s in seeds:
         mol, pf = molecule_setup(n_atoms, s, tau)
         pp, t, _, total = sample_targets(mol, n_samples // len(seeds), seed=s + 7, tau=tau)
-        feats.append(mlp.quartet_features_np(pf, pp).astype(np.float32))
-        targets.append(t.astype(np.float32))
+        F = mlp.quartet_features_np(pf, pp).astype(np.float32)
+        feats.append(F)
+        targets.append(np.clip(t - F[:, 11], -25.0, 0.0).astype(np.float32))
         if log:
             log("training molecule seed %d: %d screened quartets, sampled %d" % (s, total, len(pp)))
     return np.concatenate(feats), np.concatenate(targets)
@@ -104,7 +107,7 @@ class LearnedRouter:
 
 
 def build_router(n_train=2_000_000, train_seeds=(202611, 202612), val_seed=202619, n_val=400_000, n_atoms=200,
-                 tau=1e-12, tau_skip=5e-11, fp32_budget=1e-11, epochs=10, log=None):
+                 tau=1e-12, tau_skip=5e-11, fp32_budget=1e-11, epochs=10, under_weight=10.0, log=None):
     """Train and calibrate a router on molecules of their own seeds."""
     import time
     import torch
@@ -117,7 +120,8 @@ def build_router(n_train=2_000_000, train_seeds=(202611, 202612), val_seed=20261
     Xs = np.zeros((len(X), mlp.KF), dtype=np.float32)
     Xs[:, :mlp.NF] = (X - mean) / scale
     t0 = time.perf_counter()
-    hist = net.fit(_device(Xs, torch), _device(T, torch), epochs=epochs, batch=4096, lr=2e-3, log=log)
+    hist = net.fit(_device(Xs, torch), _device(T, torch), epochs=epochs, batch=4096, lr=2e-3, log=log,
+                   under_weight=under_weight)
     torch.cuda.synchronize()
     t_train = time.perf_counter() - t0
     # calibration on a held-out molecule

