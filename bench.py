@@ -85,7 +85,9 @@ def run_config3(args):
     t_router = time.perf_counter() - t0
     t0 = time.perf_counter()
     mol, pf = screening.molecule_setup(args.atoms, 202603)
-    pp = mol.screened_quartets(1e-12)
+    # quartets listed in the plan's order (class, primitive count): route codes
+    # and skipped output blocks are then read and written sequentially
+    pp = screening.plan_sorted(mol, mol.screened_quartets(1e-12))
     batch = mol.pair_batch(pp)
     t_setup = time.perf_counter() - t0
     log("config3 molecule: %d atoms, %d shells, %d pairs, %d screened quartets, %d integrals (%.1fs)"

@@ -120,8 +120,13 @@ int intc_plan_graph_launch(intc_plan *plan, void *stream);
  * route code 1 as FP64 and launches the FP64 kernels only (no effect on
  * graphs already captured). */
 int intc_plan_set_route_fp32(intc_plan *plan, int use_fp32);
-/* FP64/FP32 queue lengths per bin of the last routed execution (2 ints per bin). */
+/* FP64/FP32/skip queue lengths per bin of the last routed execution (3 ints per bin). */
 int intc_plan_route_counts(intc_plan *plan, int *counts);
+/* Plan order: order[i] = caller's index of the quartet at plan position i
+ * (class, then primitive count descending, then caller index).  A caller that
+ * lists its quartets in this order gets the identity, and routed executions
+ * then read the route array and write skipped blocks sequentially. */
+int intc_plan_quartet_order(intc_plan *plan, int *order);
 int intc_plan_destroy(intc_plan *plan);
 
 #ifdef __cplusplus

@@ -16,11 +16,14 @@
 //
 // Pair feature table (8 floats per shell pair): log10 Q (Schwarz factor),
 // log10 zeta_min, |AB|, la + lb, P_x, P_y, P_z (centre of the most diffuse
-// primitive pair), zeta_min.  Quartet features (12, standardised with the
-// model's mean/scale): log10 Q_bra, log10 Q_ket, log10 zeta_bra, log10 zeta_ket,
+// primitive pair), lowest multipole order m (|la - lb| for a one-centre pair,
+// 0 otherwise).  Quartet features (12, standardised with the model's
+// mean/scale): log10 Q_bra, log10 Q_ket, log10 zeta_bra, log10 zeta_ket,
 // |AB|, |CD|, L_bra, L_ket, R_PQ, log10 rho, s-type Coulomb decay
 // log10[F0(rho R_PQ^2) (4 zeta_bra zeta_ket)^(1/4) / sqrt(zeta_bra + zeta_ket)],
-// log10 Q_bra + log10 Q_ket.
+// multipole decay -(m_bra + m_ket) log10(1 + R_PQ sqrt(rho)) (an order-m
+// distribution of width ~rho^(-1/2) interacts ~(rho^(-1/2) / R)^m more weakly
+// than a monopole once R exceeds the width).
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <algorithm>
@@ -49,13 +52,13 @@ __device__ __forceinline__ uint16_t f2bf(float f)
 
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
-// 12 standardised features of quartet (pb | pk) into f[0..11]
+// 12 unstandardised features of quartet (pb | pk) into f[0..11]
 __device__ __forceinline__ void quartet_features(const float *__restrict__ pf, int pb, int pk, float *f)
 {
     const float *a = pf + 8 * (size_t)pb, *c = pf + 8 * (size_t)pk;
     float dx = a[4] - c[4], dy = a[5] - c[5], dz = a[6] - c[6];
     float r2 = dx * dx + dy * dy + dz * dz;
-    float za = a[7], zc = c[7];
+    float za = exp10f(a[1]), zc = exp10f(c[1]);
     float rho = za * zc / (za + zc);
     f[0] = a[0];
     f[1] = c[0];
@@ -65,14 +68,15 @@ __device__ __forceinline__ void quartet_features(const float *__restrict__ pf, i
     f[5] = c[2];
     f[6] = a[3];
     f[7] = c[3];
-    f[8] = sqrtf(r2);
+    const float R = sqrtf(r2);
+    f[8] = R;
     f[9] = log10f(rho);
     // s-type Coulomb decay below the Schwarz bound for the diffuse pairs:
     // (ab|cd) / sqrt((ab|ab)(cd|cd)) = F0(T) (2 za)^(1/4) (2 zc)^(1/4) / sqrt(za + zc)
     const float T = rho * r2;
     const float f0 = T < 1e-6f ? 1.0f : 0.886226925f * erff(sqrtf(T)) / sqrtf(T);
     f[10] = log10f(f0) + 0.25f * log10f(4.0f * za * zc) - 0.5f * log10f(za + zc);
-    f[11] = a[0] + c[0];
+    f[11] = -(a[7] + c[7]) * log10f(1.0f + R * sqrtf(rho));
 }
 
 }  // namespace
@@ -93,7 +97,9 @@ __global__ void __launch_bounds__(256) intc_mlp_features_k(long long n, const fl
 }
 
 // ---------------------------------------------------------------------------
-// inference + routing on MFMA.  One wave per tile of 16 quartets.
+// inference + routing on MFMA.  One wave per super-tile of 64 quartets: lane
+// l computes (once) the standardised features of quartet 64 t + l into LDS,
+// then the wave runs the network on the four 16-quartet tiles.
 // v_mfma_f32_16x16x16_bf16 operands (lane l, slot j = 0..3):
 //   A[i][k]: i = l % 16, k = 4 (l / 16) + j
 //   B[k][n]: n = l % 16, k = 4 (l / 16) + j
@@ -101,6 +107,8 @@ __global__ void __launch_bounds__(256) intc_mlp_features_k(long long n, const fl
 // route: 2 if pred + m_skip < log_skip, else 1 if pred + m_32 < log_32 and
 // log10 Q_bra + log10 Q_ket < log_qq32, else 0.
 // ---------------------------------------------------------------------------
+constexpr int SF = KF + 1;  // LDS row: 16 standardised features (zero padded) + log10 Q_bra Q_ket
+
 __global__ void __launch_bounds__(256) intc_mlp_route_k(long long n, const float *__restrict__ pf,
                                                         const int *__restrict__ qp, const float *__restrict__ norm,
                                                         const uint16_t *__restrict__ wb,
@@ -111,6 +119,7 @@ __global__ void __launch_bounds__(256) intc_mlp_route_k(long long n, const float
     __shared__ uint16_t sW[NB16];
     __shared__ float sB[2 * NH + NH + 1];
     __shared__ float sH[4][16][NH + 1];
+    __shared__ float sF[4][64][SF];
     for (int i = threadIdx.x; i < NB16; i += blockDim.x) sW[i] = wb[i];
     for (int i = threadIdx.x; i < NH; i += blockDim.x) {
         sB[i] = master[OB1 + i];
@@ -129,79 +138,84 @@ __global__ void __launch_bounds__(256) intc_mlp_route_k(long long n, const float
             for (int j = 0; j < 4; ++j)
                 b2w[ks][nt][j] = (short)sW[BW2 + (nt * 16 + r16) * NH + ks * 16 + 4 * g4 + j];
     }
-    const long long ntile = (n + 15) / 16;
+    const long long nsuper = (n + 63) / 64;
     const long long stride = (long long)gridDim.x * (blockDim.x >> 6);
-    for (long long t = (long long)blockIdx.x * (blockDim.x >> 6) + wave; t < ntile; t += stride) {
-        // layer-1 A operand: row r16, features 4*g4 .. 4*g4+3
-        long long q = t * 16 + r16;
-        long long qc = q < n ? q : n - 1;
-        float f[NF];
-        quartet_features(pf, qp[2 * qc], qp[2 * qc + 1], f);
-        bf16x4 a1;
-        for (int j = 0; j < 4; ++j) {
-            int k = 4 * g4 + j;
-            float v = k < NF ? (f[k] - norm[k]) * norm[NF + k] : 0.0f;
-            a1[j] = (short)f2bf(v);
-        }
-        const float logqq = f[11];
-        // layer 1
-        for (int nt = 0; nt < 4; ++nt) {
-            f32x4 d = {0.f, 0.f, 0.f, 0.f};
-            d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a1, b1w[nt], d, 0, 0, 0);
-            for (int j = 0; j < 4; ++j) {
-                float h = d[j] + sB[nt * 16 + r16];
-                sH[wave][4 * g4 + j][nt * 16 + r16] = h > 0.f ? h : 0.f;
-            }
+    for (long long t = (long long)blockIdx.x * (blockDim.x >> 6) + wave; t < nsuper; t += stride) {
+        {
+            const long long q = t * 64 + lane;
+            const long long qc = q < n ? q : n - 1;
+            float f[NF];
+            quartet_features(pf, qp[2 * qc], qp[2 * qc + 1], f);
+            for (int k = 0; k < NF; ++k) sF[wave][lane][k] = (f[k] - norm[k]) * norm[NF + k];
+            for (int k = NF; k < KF; ++k) sF[wave][lane][k] = 0.0f;
+            sF[wave][lane][KF] = f[0] + f[1];
         }
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         __builtin_amdgcn_wave_barrier();
-        // layer 2: K = 64 in 4 steps
-        f32x4 acc[4];
-        for (int nt = 0; nt < 4; ++nt) acc[nt] = f32x4{0.f, 0.f, 0.f, 0.f};
-        for (int ks = 0; ks < 4; ++ks) {
-            bf16x4 a;
-            for (int j = 0; j < 4; ++j) a[j] = (short)f2bf(sH[wave][r16][ks * 16 + 4 * g4 + j]);
-            for (int nt = 0; nt < 4; ++nt)
-                acc[nt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b2w[ks][nt], acc[nt], 0, 0, 0);
-        }
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        // layer 3: rows 4*g4 + j, columns r16 + 16 nt, reduced over r16
-        float part[4];
-        for (int j = 0; j < 4; ++j) {
-            float s = 0.f;
+        for (int tt = 0; tt < 4; ++tt) {
+            const long long row0 = t * 64 + 16 * tt;
+            if (row0 >= n) break;  // uniform over the wave
+            // layer-1 A operand: row r16 of this tile, features 4*g4 .. 4*g4+3
+            bf16x4 a1;
+            for (int j = 0; j < 4; ++j) a1[j] = (short)f2bf(sF[wave][16 * tt + r16][4 * g4 + j]);
+            // layer 1
             for (int nt = 0; nt < 4; ++nt) {
-                int col = nt * 16 + r16;
-                float h = acc[nt][j] + sB[NH + col];
-                h = h > 0.f ? h : 0.f;
-                s += bf2f(f2bf(h)) * bf2f(sW[BW3 + col]);
+                f32x4 d = {0.f, 0.f, 0.f, 0.f};
+                d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a1, b1w[nt], d, 0, 0, 0);
+                for (int j = 0; j < 4; ++j) {
+                    float h = d[j] + sB[nt * 16 + r16];
+                    sH[wave][4 * g4 + j][nt * 16 + r16] = h > 0.f ? h : 0.f;
+                }
             }
-            part[j] = s;
-        }
-        for (int off = 1; off < 16; off <<= 1)
-            for (int j = 0; j < 4; ++j) part[j] += __shfl_xor(part[j], off);
-        // lane with r16 == j writes row 4*g4 + j; its own quartet features are
-        // for row r16, so fetch log Q product of that row from lane 4*g4 + j
-        for (int j = 0; j < 4; ++j) {
-            int row = 4 * g4 + j;
-            float lq = __shfl(logqq, row);
-            if (r16 == j) {
-                long long qq = t * 16 + row;
-                if (qq < n) {
-                    // the network predicts log10(max|integral| / (Q_bra Q_ket)) <= 0
-                    float y = lq + fminf(part[j] + sB[3 * NH], 0.0f);
-                    if (pred) pred[qq] = y;
-                    if (route) {
-                        signed char r = 0;
-                        if (y + m_skip < log_skip) r = 2;
-                        else if (y + m_32 < log_32 && lq < log_qq32) r = 1;
-                        route[qq] = r;
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            // layer 2: K = 64 in 4 steps
+            f32x4 acc[4];
+            for (int nt = 0; nt < 4; ++nt) acc[nt] = f32x4{0.f, 0.f, 0.f, 0.f};
+            for (int ks = 0; ks < 4; ++ks) {
+                bf16x4 a;
+                for (int j = 0; j < 4; ++j) a[j] = (short)f2bf(sH[wave][r16][ks * 16 + 4 * g4 + j]);
+                for (int nt = 0; nt < 4; ++nt)
+                    acc[nt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b2w[ks][nt], acc[nt], 0, 0, 0);
+            }
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            // layer 3: rows 4*g4 + j, columns r16 + 16 nt, reduced over r16
+            float part[4];
+            for (int j = 0; j < 4; ++j) {
+                float sacc = 0.f;
+                for (int nt = 0; nt < 4; ++nt) {
+                    int col = nt * 16 + r16;
+                    float h = acc[nt][j] + sB[NH + col];
+                    h = h > 0.f ? h : 0.f;
+                    sacc += bf2f(f2bf(h)) * bf2f(sW[BW3 + col]);
+                }
+                part[j] = sacc;
+            }
+            for (int off = 1; off < 16; off <<= 1)
+                for (int j = 0; j < 4; ++j) part[j] += __shfl_xor(part[j], off);
+            // the lane with r16 == j writes row 4*g4 + j
+            for (int j = 0; j < 4; ++j) {
+                const int row = 4 * g4 + j;
+                if (r16 == j) {
+                    const long long qq = row0 + row;
+                    if (qq < n) {
+                        const float lq = sF[wave][16 * tt + row][KF];
+                        // the network predicts log10(max|integral| / (Q_bra Q_ket)) <= 0
+                        const float y = lq + fminf(part[j] + sB[3 * NH], 0.0f);
+                        if (pred) pred[qq] = y;
+                        if (route) {
+                            signed char r = 0;
+                            if (y + m_skip < log_skip) r = 2;
+                            else if (y + m_32 < log_32 && lq < log_qq32) r = 1;
+                            route[qq] = r;
+                        }
                     }
                 }
             }
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            __builtin_amdgcn_wave_barrier();
         }
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        __builtin_amdgcn_wave_barrier();
     }
 }
 
@@ -345,8 +359,8 @@ INTC_API int intc_mlp_route(long long n, const float *pair_feat, const int *quar
                             float log_32, float log_qq32, float *pred, signed char *route, void *stream)
 {
     if (n <= 0) return 0;
-    long long tiles = (n + 15) / 16;
-    int grid = (int)std::min<long long>((tiles + 3) / 4, 256 * 16);
+    long long supers = (n + 63) / 64;
+    int grid = (int)std::min<long long>((supers + 3) / 4, 256 * 12);
     hipLaunchKernelGGL(intc_mlp_route_k, dim3(grid), dim3(256), 0, (hipStream_t)stream, n, pair_feat,
                        quartet_pairs, norm, wb, master, m_skip, log_skip, m_32, log_32, log_qq32, pred, route);
     return hipGetLastError() == hipSuccess ? 0 : -3;

@@ -169,9 +169,15 @@ struct intc_plan {
     int *pair_sa = nullptr, *pair_sb = nullptr, *pair_off = nullptr, *pair_np = nullptr, *pair_nk = nullptr;
     int *pp_pair = nullptr;
     int *shell_off = nullptr;
-    // routed queues: per bin, FP64 and FP32 index lists (segments at q0) and counts
-    int *idx64 = nullptr, *idx32 = nullptr, *counts = nullptr;
+    // routed queues: per bin, FP64, FP32 and skip index lists (segments at q0)
+    // and their lengths (3 per bin)
+    int *idx64 = nullptr, *idx32 = nullptr, *idxs = nullptr, *counts = nullptr;
     int *bin_q0 = nullptr;
+    // route-compaction tiles (consecutive plan positions inside one bin)
+    int n_tile = 0;
+    int *tile_q0 = nullptr, *tile_n = nullptr, *tile_bin = nullptr, *bin_tile0 = nullptr;
+    int *tile_cnt = nullptr, *tile_base = nullptr;
+    bool identity_order = false;  // caller's quartet order is the plan order
     // device pair data (filled by execute)
     double *pz = nullptr, *po2z = nullptr, *ppx = nullptr, *ppy = nullptr, *ppz = nullptr, *pkap = nullptr;
     double *pairA = nullptr, *pairAB = nullptr;
@@ -198,7 +204,8 @@ void plan_free(intc_plan *p)
     p->side.clear();
     void *ptrs[] = {p->q_bra, p->q_ket, p->q_off, p->q_str, p->q_orig, p->q_bin, p->q_size, p->pair_sa,
                     p->pair_sb, p->pair_off, p->pair_np, p->pair_nk, p->pp_pair, p->shell_off, p->idx64,
-                    p->idx32, p->counts, p->bin_q0, p->pz, p->po2z, p->ppx, p->ppy, p->ppz, p->pkap,
+                    p->idx32, p->idxs, p->counts, p->tile_q0, p->tile_n, p->tile_bin, p->bin_tile0,
+                    p->tile_cnt, p->tile_base, p->bin_q0, p->pz, p->po2z, p->ppx, p->ppy, p->ppz, p->pkap,
                     p->pairA, p->pairAB};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
@@ -238,59 +245,160 @@ int grid_for(const IntcBin &b, int prec)
     return std::max(1, std::min((b.nq + teams_per_block - 1) / teams_per_block, 256 * 32));
 }
 
-// Route compaction: for every quartet (plan order) read its route code
-// (0 FP64, 1 FP32, 2 skip) and append its position within its bin to the
-// bin's FP64 or FP32 queue; skipped quartets get their output block zeroed.  Positions are
-// claimed per wave (ballot prefix + one atomic per wave and code), so the
-// queues keep roughly the plan's primitive-count order.
-__global__ void __launch_bounds__(256) intc_route_compact(
-    long long nq, const signed char *__restrict__ route, const int *__restrict__ q_orig,
-    const int *__restrict__ q_bin, const int *__restrict__ bin_q0, const long long *__restrict__ q_off,
-    const int *__restrict__ q_size, int *__restrict__ counts, int *__restrict__ idx64, int *__restrict__ idx32,
-    double *__restrict__ out, int use32)
+// Route compaction, stable and without atomics: the plan's quartets are cut
+// into tiles of RT_TILE consecutive plan positions that never straddle a bin.
+//   intc_route_count    per tile: how many quartets take each route code
+//   intc_route_scan     per bin: exclusive scan of the tile counts -> each
+//                       tile's first queue slot per code, and the bin totals
+//   intc_route_scatter  per tile: every quartet's position within its bin,
+//                       appended to the bin's FP64, FP32 or skip queue
+// The queues list positions in plan order (primitive count descending), the
+// same order the unrouted launch walks.  Route codes: 0 FP64, 1 FP32 (read as
+// 0 when use32 is off), 2 skip; anything else reads as 0.  When the caller's
+// quartet order is the plan order (q_orig is the identity) route[] is read
+// directly, otherwise through q_orig.
+constexpr int RT_TILE = 2048, RT_PER_THREAD = RT_TILE / 256;
+
+__device__ __forceinline__ int route_code(const signed char *__restrict__ route, const int *__restrict__ q_orig,
+                                          int identity, int i, int use32)
 {
-    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const bool live = i < nq;
-    int r = 2, b = 0;
-    if (live) {
-        r = route[q_orig[i]];
-        b = q_bin[i];
-        if (r < 0 || r > 2) r = 0;
-        if (r == 1 && !use32) r = 0;
-    }
-    // a wave's quartets are consecutive in plan order and mostly share a bin;
-    // lanes whose bin differs from lane 0's take the slow path
-    const int lane = threadIdx.x & 63;
-    const int b0 = __shfl(b, 0);
-    const bool same = live && b == b0;
-    for (int code = 0; code < 2; ++code) {
-        const bool want = same && r == code;
-        const unsigned long long m = __ballot(want);
-        if (m == 0ull) continue;
-        const int leader = __ffsll((long long)m) - 1;
-        int base = 0;
-        if (lane == leader) base = atomicAdd(&counts[2 * b0 + code], __popcll(m));
-        base = __shfl(base, leader);
-        if (want) {
-            const int pos = base + __popcll(m & ((1ull << lane) - 1ull));
-            // queue entries are positions relative to the bin's first quartet
-            (code == 0 ? idx64 : idx32)[bin_q0[b0] + pos] = (int)(i - bin_q0[b0]);
+    int r = route[identity ? i : q_orig[i]];
+    if (r < 0 || r > 2) r = 0;
+    if (r == 1 && !use32) r = 0;
+    return r;
+}
+
+// block-wide exclusive scan of three counters per thread (256 threads);
+// returns the block totals in tot[]
+__device__ __forceinline__ void block_scan3(int v[3], int ex[3], int tot[3])
+{
+    __shared__ int wsum[4][3];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int inc[3] = {v[0], v[1], v[2]};
+    for (int off = 1; off < 64; off <<= 1)
+        for (int c = 0; c < 3; ++c) {
+            int u = __shfl_up(inc[c], off);
+            if (lane >= off) inc[c] += u;
         }
+    if (lane == 63)
+        for (int c = 0; c < 3; ++c) wsum[wave][c] = inc[c];
+    __syncthreads();
+    for (int c = 0; c < 3; ++c) {
+        int before = 0, all = 0;
+        for (int w = 0; w < 4; ++w) {
+            if (w < wave) before += wsum[w][c];
+            all += wsum[w][c];
+        }
+        ex[c] = before + inc[c] - v[c];
+        tot[c] = all;
     }
-    if (live && !same && r < 2) {
-        const int pos = atomicAdd(&counts[2 * b + r], 1);
-        (r == 0 ? idx64 : idx32)[bin_q0[b] + pos] = (int)(i - bin_q0[b]);
+    __syncthreads();
+}
+
+__global__ void __launch_bounds__(256) intc_route_count(const signed char *__restrict__ route,
+                                                       const int *__restrict__ q_orig, int identity,
+                                                       const int *__restrict__ tile_q0,
+                                                       const int *__restrict__ tile_n, int use32,
+                                                       int *__restrict__ tile_cnt)
+{
+    const int t = blockIdx.x, q0 = tile_q0[t], n = tile_n[t];
+    int v[3] = {0, 0, 0};
+    for (int k = 0; k < RT_PER_THREAD; ++k) {
+        const int j = threadIdx.x * RT_PER_THREAD + k;
+        if (j < n) ++v[route_code(route, q_orig, identity, q0 + j, use32)];
     }
-    if (live && r == 2) {
-        double *o = out + q_off[i];
-        for (int k = 0; k < q_size[i]; ++k) o[k] = 0.0;
+    int ex[3], tot[3];
+    block_scan3(v, ex, tot);
+    if (threadIdx.x == 0)
+        for (int c = 0; c < 3; ++c) tile_cnt[3 * t + c] = tot[c];
+}
+
+__global__ void __launch_bounds__(256) intc_route_scan(const int *__restrict__ bin_tile0,
+                                                      const int *__restrict__ tile_cnt,
+                                                      int *__restrict__ tile_base, int *__restrict__ counts)
+{
+    const int b = blockIdx.x, t0 = bin_tile0[b], nt = bin_tile0[b + 1] - t0;
+    int run[3] = {0, 0, 0};
+    for (int base = 0; base < nt; base += 256) {
+        const int i = base + threadIdx.x;
+        int v[3], ex[3], tot[3];
+        for (int c = 0; c < 3; ++c) v[c] = i < nt ? tile_cnt[3 * (t0 + i) + c] : 0;
+        block_scan3(v, ex, tot);
+        if (i < nt)
+            for (int c = 0; c < 3; ++c) tile_base[3 * (t0 + i) + c] = run[c] + ex[c];
+        for (int c = 0; c < 3; ++c) run[c] += tot[c];
+    }
+    if (threadIdx.x == 0)
+        for (int c = 0; c < 3; ++c) counts[3 * b + c] = run[c];
+}
+
+__global__ void __launch_bounds__(256) intc_route_scatter(const signed char *__restrict__ route,
+                                                         const int *__restrict__ q_orig, int identity,
+                                                         const int *__restrict__ tile_q0,
+                                                         const int *__restrict__ tile_n,
+                                                         const int *__restrict__ tile_bin,
+                                                         const int *__restrict__ bin_q0,
+                                                         const int *__restrict__ tile_base, int use32,
+                                                         int *__restrict__ idx64, int *__restrict__ idx32,
+                                                         int *__restrict__ idxs)
+{
+    const int t = blockIdx.x, q0 = tile_q0[t], n = tile_n[t];
+    const int bq0 = bin_q0[tile_bin[t]];
+    int r[RT_PER_THREAD];
+    int v[3] = {0, 0, 0};
+    for (int k = 0; k < RT_PER_THREAD; ++k) {
+        const int j = threadIdx.x * RT_PER_THREAD + k;
+        r[k] = j < n ? route_code(route, q_orig, identity, q0 + j, use32) : 3;
+        if (r[k] < 3) ++v[r[k]];
+    }
+    int ex[3], tot[3];
+    block_scan3(v, ex, tot);
+    int pos[3];
+    for (int c = 0; c < 3; ++c) pos[c] = bq0 + tile_base[3 * t + c] + ex[c];
+    for (int k = 0; k < RT_PER_THREAD; ++k) {
+        const int j = threadIdx.x * RT_PER_THREAD + k;
+        const int c = r[k];
+        if (c < 3) {
+            // queue entries are positions relative to the bin's first quartet
+            (c == 0 ? idx64 : c == 1 ? idx32 : idxs)[pos[c]++] = q0 + j - bq0;
+        }
     }
 }
 
-__global__ void intc_zero_ints(int *p, int n)
+// Zero the output blocks of skipped quartets.  blockIdx.y = bin (all blocks
+// of a bin have the same size sz).  Each wave takes 64 skip entries at a time:
+// lane j loads entry j's output offset once, then the wave walks the 64 * sz
+// flattened elements, one double per lane, fetching the entry's offset by a
+// lane shuffle.  Skipped quartets that are adjacent in the output give
+// contiguous 512-byte stores.
+__global__ void __launch_bounds__(256) intc_zero_skipped(const int *__restrict__ bin_q0,
+                                                        const int *__restrict__ counts,
+                                                        const int *__restrict__ idxs,
+                                                        const long long *__restrict__ q_off,
+                                                        const int *__restrict__ q_size, double *__restrict__ out)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = 0;
+    const int b = blockIdx.y;
+    const int n = counts[3 * b + 2];
+    if (n == 0) return;
+    const int q0 = bin_q0[b];
+    const int sz = q_size[q0];
+    const float inv = 1.0f / (float)sz;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (long long c0 = ((long long)blockIdx.x * 4 + wave) * 64; c0 < n; c0 += (long long)gridDim.x * 4 * 64) {
+        const int m = (int)min((long long)64, n - c0);
+        const long long off = lane < m ? q_off[q0 + idxs[q0 + c0 + lane]] : 0;
+        const int ne = m * sz;  // < 2^24: exact in float
+        for (int base = 0; base < ne; base += 64) {
+            const int e = base + lane;
+            int j = (int)((float)e * inv);
+            if (j * sz > e) --j;
+            if ((j + 1) * sz <= e) ++j;
+            j = min(j, m - 1);
+            // every lane takes part in the shuffle; only in-range lanes store
+            const long long oj = __shfl(off, j);
+            if (e < ne) out[oj + (e - j * sz)] = 0.0;
+        }
+    }
 }
 
 }  // namespace
@@ -369,8 +477,25 @@ static int finish_plan(intc_plan *p, const std::vector<const IntcClassEntry *> &
         p->bins.push_back(b);
         i = j;
     }
+    std::vector<int> tile_q0, tile_n, tile_bin, bin_tile0;
+    for (size_t b = 0; b < p->bins.size(); ++b) {
+        bin_tile0.push_back((int)tile_q0.size());
+        for (int s0 = 0; s0 < p->bins[b].nq; s0 += RT_TILE) {
+            tile_q0.push_back(p->bins[b].q0 + s0);
+            tile_n.push_back(std::min(RT_TILE, p->bins[b].nq - s0));
+            tile_bin.push_back((int)b);
+        }
+    }
+    bin_tile0.push_back((int)tile_q0.size());
+    p->n_tile = (int)tile_q0.size();
+    p->identity_order = true;
+    for (long long i = 0; i < nq && p->identity_order; ++i) p->identity_order = q_orig[i] == i;
     std::vector<int> soff(shell_off, shell_off + n_shell);
     int rc = 0;
+    rc |= dev_upload(&p->tile_q0, tile_q0);
+    rc |= dev_upload(&p->tile_n, tile_n);
+    rc |= dev_upload(&p->tile_bin, tile_bin);
+    rc |= dev_upload(&p->bin_tile0, bin_tile0);
     rc |= dev_upload(&p->q_bra, q_bra);
     rc |= dev_upload(&p->q_ket, q_ket);
     rc |= dev_upload(&p->q_off, q_off);
@@ -392,8 +517,10 @@ static int finish_plan(intc_plan *p, const std::vector<const IntcClassEntry *> &
     p->n_quartet = nq;
     if (nq > 0) {
         if (hipMalloc(&p->idx64, sizeof(int) * nq) != hipSuccess ||
-            hipMalloc(&p->idx32, sizeof(int) * nq) != hipSuccess ||
-            hipMalloc(&p->counts, sizeof(int) * 2 * std::max<size_t>(1, p->bins.size())) != hipSuccess)
+            hipMalloc(&p->idx32, sizeof(int) * nq) != hipSuccess || hipMalloc(&p->idxs, sizeof(int) * nq) != hipSuccess ||
+            hipMalloc(&p->counts, sizeof(int) * 3 * std::max<size_t>(1, p->bins.size())) != hipSuccess ||
+            hipMalloc(&p->tile_cnt, sizeof(int) * 3 * std::max(1, p->n_tile)) != hipSuccess ||
+            hipMalloc(&p->tile_base, sizeof(int) * 3 * std::max(1, p->n_tile)) != hipSuccess)
             return fail(-3, "hipMalloc route queues");
     }
     if (npp > 0) {
@@ -705,12 +832,18 @@ static int route_queues(intc_plan *p, const signed char *route, double *out, hip
 {
     int nb = (int)p->bins.size();
     if (p->n_quartet == 0) return 0;
-    hipLaunchKernelGGL(intc_zero_ints, dim3((2 * nb + 255) / 256), dim3(256), 0, st, p->counts, 2 * nb);
+    const int use32 = p->route_fp32 ? 1 : 0, ident = p->identity_order ? 1 : 0;
+    hipLaunchKernelGGL(intc_route_count, dim3(p->n_tile), dim3(256), 0, st, route, p->q_orig, ident, p->tile_q0,
+                       p->tile_n, use32, p->tile_cnt);
     HIPCHK(hipGetLastError());
-    int blocks = (int)((p->n_quartet + 255) / 256);
-    hipLaunchKernelGGL(intc_route_compact, dim3(blocks), dim3(256), 0, st, p->n_quartet, route, p->q_orig,
-                       p->q_bin, p->bin_q0, p->q_off, p->q_size, p->counts, p->idx64, p->idx32, out,
-                       p->route_fp32 ? 1 : 0);
+    hipLaunchKernelGGL(intc_route_scan, dim3(nb), dim3(256), 0, st, p->bin_tile0, p->tile_cnt, p->tile_base,
+                       p->counts);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(intc_route_scatter, dim3(p->n_tile), dim3(256), 0, st, route, p->q_orig, ident, p->tile_q0,
+                       p->tile_n, p->tile_bin, p->bin_q0, p->tile_base, use32, p->idx64, p->idx32, p->idxs);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(intc_zero_skipped, dim3(1024, nb), dim3(256), 0, st, p->bin_q0, p->counts, p->idxs,
+                       p->q_off, p->q_size, out);
     HIPCHK(hipGetLastError());
     return 0;
 }
@@ -745,7 +878,7 @@ static int run_all(intc_plan *p, const double *sph, double *out, const signed ch
             hipStream_t s2 = (k % n_streams) == 0 ? st : p->side[(k % n_streams) - 1];
             ++k;
             const int *qi = route ? (prec == 0 ? p->idx64 : p->idx32) + b.q0 : nullptr;
-            const int *nd = route ? p->counts + 2 * bi + prec : nullptr;
+            const int *nd = route ? p->counts + 3 * bi + prec : nullptr;
             if (route) {
                 // queue lengths are only known on the device: persistent grid
                 IntcBin br = b;
@@ -863,12 +996,20 @@ INTC_API int intc_plan_set_route_fp32(intc_plan *p, int use_fp32)
     return 0;
 }
 
+// plan order: order[i] = the caller's index of the quartet at plan position i
+INTC_API int intc_plan_quartet_order(intc_plan *p, int *order)
+{
+    if (p->n_quartet == 0) return 0;
+    HIPCHK(hipMemcpy(order, p->q_orig, sizeof(int) * p->n_quartet, hipMemcpyDeviceToHost));
+    return 0;
+}
+
 // queue lengths of the last routed execution (host copy, synchronising)
 INTC_API int intc_plan_route_counts(intc_plan *p, int *counts_out)
 {
     int nb = (int)p->bins.size();
     if (nb == 0) return 0;
-    HIPCHK(hipMemcpy(counts_out, p->counts, sizeof(int) * 2 * nb, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(counts_out, p->counts, sizeof(int) * 3 * nb, hipMemcpyDeviceToHost));
     return 0;
 }
 

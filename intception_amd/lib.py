@@ -44,6 +44,7 @@ SIGNATURES = [
     ("intc_plan_graph_launch", c_int, [c_void_p, c_void_p]),
     ("intc_plan_set_route_fp32", c_int, [c_void_p, c_int]),
     ("intc_plan_route_counts", c_int, [c_void_p, _ip]),
+    ("intc_plan_quartet_order", c_int, [c_void_p, _ip]),
     ("intc_plan_destroy", c_int, [c_void_p]),
 ]
 

@@ -5,8 +5,9 @@ FP64 and FP32-recurrence kernels.
 Host side only: the forward, backward, Adam and routing arithmetic all run in
 libintc's HIP kernels (csrc/intc_mlp.hip); torch provides device memory.
 
-* ``pair_features(mol)``: per shell pair log10 Q, log10 zeta_min, |AB|, L and
-  the centre/exponent of the most diffuse primitive pair (8 floats).
+* ``pair_features(mol)``: per shell pair log10 Q, log10 zeta_min, |AB|, L, the
+  centre of the most diffuse primitive pair and the pair's lowest multipole
+  order (8 floats).
 * ``training_set(...)``: quartets of separate molecules (their own seeds),
   targets log10(max|integral| / (Q_bra Q_ket)) from the device FP64 path (the
   decay below the Schwarz bound; the network's output is clamped at 0, so its
@@ -61,20 +62,24 @@ def _p(t):
 # features
 # ---------------------------------------------------------------------------
 def pair_features(shell_l, shell_nprim, shell_off, shell_cen, exps, pairs, Q):
-    """(n_pair, 8) float32: log10 Q, log10 zeta_min, |AB|, la+lb, P (3), zeta_min."""
+    """(n_pair, 8) float32: log10 Q, log10 zeta_min, |AB|, la+lb, P (3), and the
+    pair's lowest multipole order: |la - lb| for a one-centre pair (its charge
+    distribution has no moment below that order, e.g. a d*s pair on one atom
+    starts at the quadrupole), 0 for a two-centre pair."""
     a, b = pairs[:, 0], pairs[:, 1]
     emin = np.array([exps[o:o + n].min() for o, n in zip(shell_off, shell_nprim)])
     za, zb = emin[a], emin[b]
     z = za + zb
     A, B = shell_cen[a], shell_cen[b]
     P = (za[:, None] * A + zb[:, None] * B) / z[:, None]
+    ab = np.linalg.norm(A - B, axis=1)
     out = np.empty((len(pairs), 8), dtype=np.float32)
     out[:, 0] = np.log10(np.maximum(Q, 1e-300))
     out[:, 1] = np.log10(z)
-    out[:, 2] = np.linalg.norm(A - B, axis=1)
+    out[:, 2] = ab
     out[:, 3] = shell_l[a] + shell_l[b]
     out[:, 4:7] = P
-    out[:, 7] = z
+    out[:, 7] = np.where(ab < 1e-8, np.abs(shell_l[a] - shell_l[b]), 0)
     return out
 
 
@@ -82,14 +87,22 @@ def quartet_features_np(pf, qp):
     """Numpy twin of the device feature function (unstandardised)."""
     a, c = pf[qp[:, 0]].astype(np.float64), pf[qp[:, 1]].astype(np.float64)
     r2 = np.sum((a[:, 4:7] - c[:, 4:7]) ** 2, axis=1)
-    rho = a[:, 7] * c[:, 7] / (a[:, 7] + c[:, 7])
+    za, zc = 10.0 ** a[:, 1], 10.0 ** c[:, 1]
+    rho = za * zc / (za + zc)
     T = rho * r2
     sT = np.sqrt(np.maximum(T, 1e-300))
     from scipy.special import erf
     f0 = np.where(T < 1e-6, 1.0, 0.886226925452758 * erf(sT) / sT)
-    decay = np.log10(f0) + 0.25 * np.log10(4 * a[:, 7] * c[:, 7]) - 0.5 * np.log10(a[:, 7] + c[:, 7])
-    return np.stack([a[:, 0], c[:, 0], a[:, 1], c[:, 1], a[:, 2], c[:, 2], a[:, 3], c[:, 3], np.sqrt(r2),
-                     np.log10(rho), decay, a[:, 0] + c[:, 0]], axis=1)
+    decay = np.log10(f0) + 0.25 * np.log10(4 * za * zc) - 0.5 * np.log10(za + zc)
+    R = np.sqrt(r2)
+    mdecay = -(a[:, 7] + c[:, 7]) * np.log10(1.0 + R * np.sqrt(rho))
+    return np.stack([a[:, 0], c[:, 0], a[:, 1], c[:, 1], a[:, 2], c[:, 2], a[:, 3], c[:, 3], R,
+                     np.log10(rho), decay, mdecay], axis=1)
+
+
+def log_schwarz_product(F):
+    """log10(Q_bra Q_ket) from unstandardised quartet features."""
+    return F[:, 0] + F[:, 1]
 
 
 def block_log_max(batch, out):
@@ -170,7 +183,7 @@ class MagnitudeNet:
         self.norm = np.concatenate([np.zeros(NF), np.ones(NF)]).astype(np.float32)
         self._upload(m)
         self.margin_skip = 0.0
-        self.margin_32 = 0.0
+        self.margin_32 = 0.0   # the FP32 thresholds are calibrated on the predictions themselves
         self.step = 0
 
     def _upload(self, master):
@@ -273,13 +286,16 @@ class MagnitudeNet:
                                        float(self.margin_32), float(log_32), float(log_qq32), _p(pred_out),
                                        _p(route_out), self._stream()))
 
-    def calibrate(self, pred, true, quantile=1.0, pad=0.25):
-        """Margin = max (or quantile) of (true - pred) on held-out quartets,
-        plus a pad; a quartet is then treated as below a threshold only when
-        pred + margin is."""
-        d = np.asarray(true, dtype=np.float64) - np.asarray(pred, dtype=np.float64)
-        self.margin_skip = float(np.quantile(d, quantile) + pad)
-        self.margin_32 = self.margin_skip
+    def calibrate(self, pred, true, log_thresh, pad=0.25):
+        """Skip margin from held-out quartets: the smallest margin under which
+        no held-out quartet whose true magnitude reaches 10**log_thresh is
+        predicted below it (pred + margin >= log_thresh for every such
+        quartet), plus a pad.  Large under-predictions of big quartets that
+        stay far above the threshold do not widen it."""
+        pred = np.asarray(pred, dtype=np.float64)
+        need = np.asarray(true, dtype=np.float64) >= log_thresh
+        gap = (log_thresh - pred[need]).max() if need.any() else 0.0
+        self.margin_skip = float(max(0.0, gap) + pad)
         return self.margin_skip
 
     def save(self, path):

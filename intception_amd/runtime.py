@@ -123,9 +123,15 @@ class EriPlan:
 
     def route_counts(self):
         n = self._L.intc_plan_num_bins(self.handle)
-        buf = (ctypes.c_int * (2 * max(n, 1)))()
+        buf = (ctypes.c_int * (3 * max(n, 1)))()
         _lib.check(self._L.intc_plan_route_counts(self.handle, buf))
-        return np.array(buf[:2 * n]).reshape(n, 2)
+        return np.array(buf[:3 * n]).reshape(n, 3)
+
+    def quartet_order(self):
+        """order[i] = index (in the batch) of the quartet at plan position i."""
+        order = np.empty(self.batch.n_quartet, dtype=np.int32)
+        _lib.check(self._L.intc_plan_quartet_order(self.handle, order.ctypes.data_as(ctypes.POINTER(ctypes.c_int))))
+        return order
 
     def capture(self, out, n_streams=4, route=None):
         """Capture pair building (+ route compaction) + all bins into one

@@ -8,8 +8,10 @@ Pipeline (every numeric step on the device through libintc):
    log10 max|integral| from the FP64 path, 12 pair-derived features.
 3. ``MagnitudeNet`` trained on the device (BF16 weights, FP32 accumulation).
 4. Calibration on a held-out molecule:
-   * skip margin: the largest under-prediction (true - predicted) plus a pad,
-     so a quartet is skipped only when predicted + margin < log10(tau_skip);
+   * skip margin: the smallest margin under which every held-out quartet
+     whose true magnitude reaches tau_skip keeps predicted + margin >=
+     log10(tau_skip), plus a pad; a quartet is skipped only when
+     predicted + margin < log10(tau_skip);
    * FP32 guard: the largest Schwarz product log10(Q_P Q_P') under which every
      held-out quartet's FP32-recurrence error stays below the FP32 budget.
 5. Route codes per quartet on the device (0 FP64, 1 FP32, 2 skip), then the
@@ -25,6 +27,17 @@ def molecule_setup(n_atoms, seed, tau=1e-12):
     mol.schwarz()
     pf = mlp.pair_features(mol.shell_l, mol.shell_nprim, mol.shell_off, mol.shell_cen, mol.exps, mol.pairs, mol.Q)
     return mol, pf
+
+
+def plan_sorted(mol, pp):
+    """The quartet list (pair-index pairs) reordered into the execution plan's
+    order (class, then primitive count descending).  A batch listed in this
+    order makes the plan order the identity, so route codes are read and
+    skipped output blocks zeroed sequentially."""
+    plan = runtime.EriPlan(mol.pair_batch(pp))
+    order = plan.quartet_order()
+    plan.close()
+    return np.ascontiguousarray(pp[order])
 
 
 def _device(a, torch, dtype=None):
@@ -62,7 +75,7 @@ def training_set(n_samples, seeds, n_atoms=200, tau=1e-12, log=None):
         pp, t, _, total = sample_targets(mol, n_samples // len(seeds), seed=s + 7, tau=tau)
         F = mlp.quartet_features_np(pf, pp).astype(np.float32)
         feats.append(F)
-        targets.append(np.clip(t - F[:, 11], -25.0, 0.0).astype(np.float32))
+        targets.append(np.clip(t - mlp.log_schwarz_product(F), -25.0, 0.0).astype(np.float32))
         if log:
             log("training molecule seed %d: %d screened quartets, sampled %d" % (s, total, len(pp)))
     return np.concatenate(feats), np.concatenate(targets)
@@ -129,7 +142,7 @@ def build_router(n_train=2_000_000, train_seeds=(202611, 202612), val_seed=20261
     pp, tv, e32, total = sample_targets(mol, n_val, seed=val_seed + 7, tau=tau, fp32=True)
     d_pf, d_qp = _device(pf, torch), _device(pp.astype(np.int32), torch)
     pred = net.predict(d_pf, d_qp).cpu().numpy()
-    margin = net.calibrate(pred, tv)
+    margin = net.calibrate(pred, tv, np.log10(tau_skip))
     router = LearnedRouter(net, np.log10(tau_skip), fp32_budget)
     logqq = np.log10(np.maximum(mol.Q[pp[:, 0]] * mol.Q[pp[:, 1]], 1e-300))
     n32, _, _ = router.calibrate_fp32(pred, logqq, e32)

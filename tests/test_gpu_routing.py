@@ -57,10 +57,22 @@ def test_fp32_recurrences_error(gpu):
             assert np.percentile(rel, 99) < 1e-3
 
 
-def test_random_route_and_graph(gpu):
+@pytest.mark.parametrize("plan_sorted", [False, True])
+def test_random_route_and_graph(gpu, plan_sorted):
+    """Random route codes: exact FP64 / FP32 / zero blocks, queue lengths, and
+    graph replays equal to eager runs.  plan_sorted lists the quartets in the
+    plan's own order (identity permutation: route read and skipped blocks
+    zeroed in place), otherwise the route is read through the permutation."""
     import torch
     b = basis.config1(n_quartet=8000, seed=5)
+    if plan_sorted:
+        p0 = runtime.EriPlan(b)
+        order = p0.quartet_order()
+        p0.close()
+        assert sorted(order.tolist()) == list(range(b.n_quartet))
+        b = b.subset(order)
     plan = runtime.EriPlan(b)
+    assert bool(np.all(plan.quartet_order() == np.arange(b.n_quartet))) == plan_sorted
     ref64 = plan.run().cpu().numpy()
     all32 = plan.run_routed(torch.ones(b.n_quartet, dtype=torch.int8, device=plan.device)).cpu().numpy()
     rng = np.random.default_rng(1)
@@ -68,6 +80,8 @@ def test_random_route_and_graph(gpu):
     route = torch.from_numpy(codes).to(plan.device)
     out = torch.full((b.out_size,), 7.0, dtype=torch.float64, device=plan.device)
     got = plan.run_routed(route, out).cpu().numpy()
+    counts = plan.route_counts()
+    assert np.array_equal(counts.sum(0), np.bincount(codes, minlength=3))
     for q, (g, r64, r32) in enumerate(zip(_blocks(b, got), _blocks(b, ref64), _blocks(b, all32))):
         if codes[q] == 0:
             assert np.array_equal(g, r64)

@@ -19,7 +19,7 @@ def test_learned_routing_small_molecule(gpu):
                                            n_atoms=40, epochs=6)
     print("router stats:", {k: v for k, v in stats.items() if k != "train_mse"}, flush=True)
     mol, pf = screening.molecule_setup(40, 23)
-    pp = mol.screened_quartets(1e-12)
+    pp = screening.plan_sorted(mol, mol.screened_quartets(1e-12))
     b = mol.pair_batch(pp)
     plan = runtime.EriPlan(b)
     ref = plan.run()

@@ -63,7 +63,7 @@ def test_mlp_kernels_match_reference(gpu):
     pf[:, 2] = rng.uniform(0, 6, n_pair)
     pf[:, 3] = rng.integers(0, 5, n_pair)
     pf[:, 4:7] = rng.uniform(-10, 10, (n_pair, 3))
-    pf[:, 7] = 10 ** pf[:, 1]
+    pf[:, 7] = rng.integers(0, 3, n_pair) * (rng.uniform(0, 1, n_pair) < 0.3)
     qp = rng.integers(0, n_pair, (n, 2)).astype(np.int32)
     F = mlp.quartet_features_np(pf, qp)
     net = mlp.MagnitudeNet(seed=4)
@@ -80,7 +80,7 @@ def test_mlp_kernels_match_reference(gpu):
     T = rng.normal(-6, 2, size=n).astype(np.float32)
     yref, lref, gref = mlp.reference_forward_backward(master, X, T, under_weight=10.0)
     # the routing kernel reports log10 Q_bra Q_ket + min(0, network output)
-    yabs = F[:, 11] + np.minimum(yref, 0.0)
+    yabs = mlp.log_schwarz_product(F) + np.minimum(yref, 0.0)
     err = np.abs(pred - yabs)
     print("mfma forward: max |err| %.2e (|y| ~ %.1f)" % (err.max(), np.abs(yref).mean()), flush=True)
     assert err.max() < 2e-3 * (1 + np.abs(yabs).max())
