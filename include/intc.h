@@ -78,17 +78,22 @@ int intc_eri_cont_cart_main(INTC_SHELL_ARGS(a), INTC_SHELL_ARGS(b), INTC_SHELL_A
 typedef struct intc_plan intc_plan;
 
 /* Host arrays; bins quartets by canonical class and primitive count and
- * uploads the index data.  Device memory is owned by the plan. */
+ * uploads the index data.  Device memory is owned by the plan.  shell_cen,
+ * exps and coefs (host copies of what execute receives) may be NULL; when
+ * given, a pair of two shells with the same l is oriented so that the
+ * recurrences build on the shell the primitive-pair centres lie closest to
+ * (better conditioned HRR for diffuse/tight pairs far apart). */
 int intc_plan_create(intc_plan **plan, int spherical, int n_shell, const int *shell_l, const int *shell_nprim,
-                     const int *shell_off, const double *shell_cen, long long n_quartet, const int *quartets,
-                     const long long *out_off);
+                     const int *shell_off, const double *shell_cen, const double *exps, const double *coefs,
+                     long long n_quartet, const int *quartets, const long long *out_off);
 /* Same, with shell-pair data shared between quartets: pair P is shells
  * (pair_shells[2P], pair_shells[2P+1]); quartet q is (pair quartet_pairs[2q] |
  * pair quartet_pairs[2q+1]) with a row-major block over those four shells in
  * that order at out[out_off[q]]. */
 int intc_plan_create_pairs(intc_plan **plan, int spherical, int n_shell, const int *shell_l, const int *shell_nprim,
-                           const int *shell_off, int n_pair, const int *pair_shells, long long n_quartet,
-                           const int *quartet_pairs, const long long *out_off);
+                           const int *shell_off, const double *shell_cen, const double *exps, const double *coefs,
+                           int n_pair, const int *pair_shells, long long n_quartet, const int *quartet_pairs,
+                           const long long *out_off);
 int intc_plan_num_bins(const intc_plan *plan);
 /* prec: 0 = FP64 kernel, 1 = kernel with FP32 recurrences (FP64 Boys,
  * prefactors, contraction sums and contracted HRR/transform). */

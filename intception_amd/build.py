@@ -61,12 +61,14 @@ def build(lmax=2, jobs=None, verbose=True):
     if not os.path.exists(inc):
         boys_tables.write_inc(inc)
     classes = codegen.class_set(lmax, EXTRA_CLASSES)
-    files, metas = codegen.generate(GEN, classes)
+    wg = codegen.wg_classes(3, classes)
+    files, metas = codegen.generate(GEN, classes, wg=wg)
     common = [os.path.join(CSRC, "intc_common.h")]
     srcs = [(f, common) for f in files]
     srcs.append((os.path.join(CSRC, "intc_runtime.hip"),
                  common + [inc, os.path.join(GEN, "class_table.inc")]))
     srcs.append((os.path.join(CSRC, "intc_mlp.hip"), []))
+    srcs.append((os.path.join(CSRC, "intc_eri_wg.hip"), common))
     jobs = jobs or min(16, os.cpu_count() or 4)
     with ThreadPoolExecutor(jobs) as ex:
         res = list(ex.map(lambda a: _compile(*a), srcs))
@@ -77,6 +79,8 @@ def build(lmax=2, jobs=None, verbose=True):
     if r.returncode != 0:
         raise RuntimeError("link failed:\n" + r.stderr[-4000:])
     os.replace(LIB + ".tmp", LIB)
+    if verbose and wg:
+        print("workgroup kernels: %d classes x 2 outputs x 2 precisions" % len(wg))
     if verbose:
         print("libintc.so: %d classes x 2 outputs x 2 precisions, %d objects rebuilt" % (len(classes), rebuilt))
     return LIB

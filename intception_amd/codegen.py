@@ -452,7 +452,19 @@ def class_set(lmax_pair_side=2, extra=()):
     return sorted(out)
 
 
-def generate(outdir, classes, sph_modes=(False, True), precs=("d", "f")):
+def wg_classes(lmax=3, covered=()):
+    """Canonical 4-centre classes with shells up to lmax that have no register
+    kernel (an f shell makes the bra group or the per-lane ket table too
+    large): they run on the workgroup-per-quartet kernels."""
+    cov = set(covered)
+    return [c for c in canonical_classes(lmax) if c not in cov]
+
+
+def _nout(l, sph):
+    return 2 * l + 1 if sph else (l + 1) * (l + 2) // 2
+
+
+def generate(outdir, classes, sph_modes=(False, True), precs=("d", "f"), wg=()):
     os.makedirs(outdir, exist_ok=True)
     metas = []
     written = []
@@ -472,15 +484,30 @@ def generate(outdir, classes, sph_modes=(False, True), precs=("d", "f")):
     for m in metas:
         lines.append('extern "C" hipError_t %s_launch(const IntcEriArgs *, int, int, size_t, hipStream_t);'
                      % m["name"])
-    lines.append("struct IntcClassEntry { int la, lb, lc, ld, sph, prec, G, NE, NCD, M; intc_launch_fn fn; };")
+    for pr in precs:
+        if wg:
+            lines.append('extern "C" hipError_t intc_eri_wg_%s_launch(const IntcEriArgs *, int, int, size_t, '
+                         'hipStream_t);' % pr)
+    # G = 64 and one group per team for the workgroup kernels (wg = 1): one
+    # quartet per 256-thread block, LDS sized by the launcher
+    lines.append("struct IntcClassEntry { int la, lb, lc, ld, sph, prec, G, NE, NCD, M, wg; intc_launch_fn fn; };")
     lines.append("static const IntcClassEntry intc_class_table[] = {")
+    n = 0
     for m in metas:
         la, lb, lc, ld = m["cls"]
-        lines.append("    {%d, %d, %d, %d, %d, %d, %d, %d, %d, %d, %s_launch}," % (
+        lines.append("    {%d, %d, %d, %d, %d, %d, %d, %d, %d, %d, 0, %s_launch}," % (
             la, lb, lc, ld, int(m["spherical"]), 0 if m["prec"] == "d" else 1, m["G"], m["NE"], m["NCD"], m["M"],
             m["name"]))
+        n += 1
+    for cls in wg:
+        la, lb, lc, ld = cls
+        for sph, pr in itertools.product(sph_modes, precs):
+            lines.append("    {%d, %d, %d, %d, %d, %d, 64, %d, %d, %d, 1, intc_eri_wg_%s_launch}," % (
+                la, lb, lc, ld, int(sph), 0 if pr == "d" else 1, _nout(la, sph) * _nout(lb, sph),
+                _nout(lc, sph) * _nout(ld, sph), la + lb + lc + ld, pr))
+            n += 1
     lines.append("};")
-    lines.append("static const int intc_n_class = %d;" % len(metas))
+    lines.append("static const int intc_n_class = %d;" % n)
     tab = os.path.join(outdir, "class_table.inc")
     src = "\n".join(lines) + "\n"
     old = open(tab).read() if os.path.exists(tab) else None

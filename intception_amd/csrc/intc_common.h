@@ -42,6 +42,7 @@ struct IntcEriArgs {
     const double *boys_small;         // [151][33]
     const double *boys_medium;        // [201][7]
     double *out;
+    int wg_cls[6];                    // workgroup kernels: la, lb, lc, ld, spherical (canonical class)
 };
 
 // offset of S_l inside IntcEriArgs::sph: sum_{l' < l} ncc(l') (2l' + 1)

@@ -47,6 +47,13 @@ struct DevTables {
 std::mutex g_mu;
 std::vector<DevTables> g_tables;  // per device
 
+}  // namespace
+
+// workgroup kernels (intc_eri_wg.hip): allow their dynamic LDS on this device
+int intc_wg_prepare();
+
+namespace {
+
 int ensure_tables(int dev, DevTables **out)
 {
     std::lock_guard<std::mutex> lk(g_mu);
@@ -59,6 +66,7 @@ int ensure_tables(int dev, DevTables **out)
         HIPCHK(hipMemcpy(t.cheb, intc_boys_cheb_host, sizeof(intc_boys_cheb_host), hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(t.small, intc_boys_smallx_host, sizeof(intc_boys_smallx_host), hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(t.medium, intc_boys_mediumx_host, sizeof(intc_boys_mediumx_host), hipMemcpyHostToDevice));
+        if (intc_wg_prepare()) return fail(-3, "workgroup kernels: cannot enable their LDS size");
     }
     *out = &t;
     return 0;
@@ -569,11 +577,41 @@ static std::vector<int> plan_order(const std::vector<const IntcClassEntry *> &cl
     return order;
 }
 
+// Orientation of a pair of two shells with the same l (the class, and so the
+// kernel, is the same either way round).  The kernels build angular momentum
+// on the pair's first centre and move it to the second with the HRR over
+// A - B; when the primitive-pair centres P lie far from the first centre (a
+// diffuse shell next to a tight one on another atom) that transfer cancels
+// many digits.  True when the second shell should come first: the primitive
+// pairs, weighted by |c_a c_b| exp(-mu R^2) / (alpha + beta), lie closer to
+// it (|P - A| = beta/(alpha+beta) R, |P - B| = alpha/(alpha+beta) R).
+static bool vrr_on_second(int a, int b, const int *shell_nprim, const int *shell_off, const double *shell_cen,
+                          const double *exps, const double *coefs)
+{
+    if (!shell_cen || !exps || !coefs || a == b) return false;
+    double r2 = 0.0;
+    for (int d = 0; d < 3; ++d) {
+        const double x = shell_cen[3 * a + d] - shell_cen[3 * b + d];
+        r2 += x * x;
+    }
+    if (r2 == 0.0) return false;
+    double wa = 0.0, wb = 0.0;
+    for (int i = 0; i < shell_nprim[a]; ++i)
+        for (int j = 0; j < shell_nprim[b]; ++j) {
+            const double al = exps[shell_off[a] + i], be = exps[shell_off[b] + j], oz = 1.0 / (al + be);
+            const double w = std::fabs(coefs[shell_off[a] + i] * coefs[shell_off[b] + j]) *
+                             std::exp(-al * be * oz * r2) * oz;
+            wa += w * be * be * oz * oz;
+            wb += w * al * al * oz * oz;
+        }
+    return wb < wa;
+}
+
 INTC_API int intc_plan_create(intc_plan **out, int spherical, int n_shell, const int *shell_l,
                               const int *shell_nprim, const int *shell_off, const double *shell_cen,
-                              long long n_quartet, const int *quartets, const long long *out_off)
+                              const double *exps, const double *coefs, long long n_quartet, const int *quartets,
+                              const long long *out_off)
 {
-    (void)shell_cen;
     *out = nullptr;
     if (n_quartet < 0) return fail(-1, "negative size");
     if (n_quartet > 0x3fffffffLL) return fail(-1, "too many quartets for one plan");
@@ -593,6 +631,11 @@ INTC_API int intc_plan_create(intc_plan **out, int spherical, int n_shell, const
         canonical(l, c, perm);
         cls[q] = find_class(c[0], c[1], c[2], c[3], spherical ? 1 : 0, 0);
         if (!cls[q]) return class_error(l[0], l[1], l[2], l[3], spherical);
+        const int *qs = quartets + 4 * (size_t)q;
+        for (int h = 0; h < 4; h += 2)
+            if (c[h] == c[h + 1] &&
+                vrr_on_second(qs[perm[h]], qs[perm[h + 1]], shell_nprim, shell_off, shell_cen, exps, coefs))
+                std::swap(perm[h], perm[h + 1]);
         for (int k = 0; k < 4; ++k) perms[q][k] = perm[k];
         int pr = 1;
         for (int k = 0; k < 4; ++k) pr *= shell_nprim[quartets[4 * (size_t)q + k]];
@@ -645,9 +688,9 @@ INTC_API int intc_plan_create(intc_plan **out, int spherical, int n_shell, const
 // row-major over the four shells in that order.  Pair data are built once per
 // pair and shared by every quartet that uses it.
 INTC_API int intc_plan_create_pairs(intc_plan **out, int spherical, int n_shell, const int *shell_l,
-                                    const int *shell_nprim, const int *shell_off, int n_pair,
-                                    const int *pair_shells, long long n_quartet, const int *quartet_pairs,
-                                    const long long *out_off)
+                                    const int *shell_nprim, const int *shell_off, const double *shell_cen,
+                                    const double *exps, const double *coefs, int n_pair, const int *pair_shells,
+                                    long long n_quartet, const int *quartet_pairs, const long long *out_off)
 {
     *out = nullptr;
     if (n_quartet < 0 || n_pair < 0) return fail(-1, "negative size");
@@ -660,7 +703,9 @@ INTC_API int intc_plan_create_pairs(intc_plan **out, int spherical, int n_shell,
     for (int k = 0; k < n_pair; ++k) {
         int a = pair_shells[2 * (size_t)k], b = pair_shells[2 * (size_t)k + 1];
         if (a < 0 || a >= n_shell || b < 0 || b >= n_shell) return fail(-1, "pair shell id out of range");
-        swapped[k] = shell_l[a] < shell_l[b];
+        swapped[k] = shell_l[a] < shell_l[b] ||
+                     (shell_l[a] == shell_l[b] &&
+                      vrr_on_second(a, b, shell_nprim, shell_off, shell_cen, exps, coefs));
         pair_sa[k] = swapped[k] ? b : a;
         pair_sb[k] = swapped[k] ? a : b;
     }
@@ -767,8 +812,14 @@ static int launch_bin(const intc_plan *p, const IntcBin &b, int prec, const Intc
     a.nq = b.nq;
     a.team_groups = b.team_groups[prec];
     const IntcClassEntry *c = b.cls[prec];
+    a.wg_cls[0] = c->la;
+    a.wg_cls[1] = c->lb;
+    a.wg_cls[2] = c->lc;
+    a.wg_cls[3] = c->ld;
+    a.wg_cls[4] = c->sph;
     int tpw = 64 / (c->G * a.team_groups);
-    size_t lds = (size_t)(b.block / 64) * tpw * c->NE * c->NCD * sizeof(double);
+    // workgroup kernels size their own LDS
+    size_t lds = c->wg ? 0 : (size_t)(b.block / 64) * tpw * c->NE * c->NCD * sizeof(double);
     hipError_t e = c->fn(&a, b.grid[prec], b.block, lds, st);
     if (e != hipSuccess) return fail(-3, std::string("class kernel launch: ") + hipGetErrorString(e));
     return 0;
@@ -1128,8 +1179,8 @@ static int per_quartet(int spherical, const ShellArg sa[4], double *output)
                     combo.push_back(ka); combo.push_back(kb); combo.push_back(kc); combo.push_back(kd);
                 }
     intc_plan *p = nullptr;
-    rc = intc_plan_create(&p, spherical, (int)sl.size(), sl.data(), snp.data(), soff.data(), cen.data(),
-                          (long long)nk, quart.data(), off.data());
+    rc = intc_plan_create(&p, spherical, (int)sl.size(), sl.data(), snp.data(), soff.data(), cen.data(), ex.data(),
+                          co.data(), (long long)nk, quart.data(), off.data());
     if (rc) return rc;
     double *d_ex = nullptr, *d_co = nullptr, *d_cen = nullptr, *d_sph = nullptr, *d_out = nullptr;
     std::vector<double> sph_all;

@@ -29,9 +29,10 @@ SIGNATURES = [
     ("intc_eri_cont_cart_work_array_size", c_int, [c_int] * 12),
     ("intc_eri_cont_sph_main", c_int, (_SHELL + [_dp]) * 4 + [_dp, _dp]),
     ("intc_eri_cont_cart_main", c_int, _SHELL * 4 + [_dp, _dp]),
-    ("intc_plan_create", c_int, [ctypes.POINTER(c_void_p), c_int, c_int, _ip, _ip, _ip, _dp, c_ll, _ip, _llp]),
-    ("intc_plan_create_pairs", c_int, [ctypes.POINTER(c_void_p), c_int, c_int, _ip, _ip, _ip, c_int, _ip, c_ll,
-                                       _ip, _llp]),
+    ("intc_plan_create", c_int, [ctypes.POINTER(c_void_p), c_int, c_int, _ip, _ip, _ip, _dp, _dp, _dp, c_ll, _ip,
+                                 _llp]),
+    ("intc_plan_create_pairs", c_int, [ctypes.POINTER(c_void_p), c_int, c_int, _ip, _ip, _ip, _dp, _dp, _dp, c_int,
+                                       _ip, c_ll, _ip, _llp]),
     ("intc_plan_num_bins", c_int, [c_void_p]),
     ("intc_plan_bin_info", c_int, [c_void_p, c_int, c_int, _ip, _ip, _ip, _ip, _ip]),
     ("intc_plan_set_bin_tuning", c_int, [c_void_p, c_int, c_int, c_int, c_int]),

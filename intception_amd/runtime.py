@@ -51,16 +51,19 @@ class EriPlan:
         h = ctypes.c_void_p()
         ip = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
         llp = oo.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong))
+        ex = np.ascontiguousarray(b.exps, dtype=np.float64)
+        co = np.ascontiguousarray(b.coefs, dtype=np.float64)
+        dp = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
         with torch.cuda.device(self.device):
             if b.pair_shells is not None:
                 ps = np.ascontiguousarray(b.pair_shells.reshape(-1), dtype=np.int32)
                 qp = np.ascontiguousarray(b.quartet_pairs.reshape(-1), dtype=np.int32)
                 _lib.check(L.intc_plan_create_pairs(ctypes.byref(h), int(b.spherical), len(sl), ip(sl), ip(snp),
-                                                    ip(soff), len(ps) // 2, ip(ps), len(oo), ip(qp), llp))
+                                                    ip(soff), dp(cen), dp(ex), dp(co), len(ps) // 2, ip(ps),
+                                                    len(oo), ip(qp), llp))
             else:
                 _lib.check(L.intc_plan_create(ctypes.byref(h), int(b.spherical), len(sl), ip(sl), ip(snp),
-                                              ip(soff), cen.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
-                                              len(oo), ip(q), llp))
+                                              ip(soff), dp(cen), dp(ex), dp(co), len(oo), ip(q), llp))
         self.handle = h
         self.d_exps = torch.from_numpy(np.ascontiguousarray(b.exps, dtype=np.float64)).to(self.device)
         self.d_coefs = torch.from_numpy(np.ascontiguousarray(b.coefs, dtype=np.float64)).to(self.device)

@@ -20,6 +20,21 @@ def have_gpu():
         return False
 
 
+@pytest.fixture(autouse=True)
+def _stack_dump_if_stuck(request):
+    """A GPU test still running after 100 s prints every thread's Python stack
+    once (names the call it waits in: a kernel, a copy or the oracle)."""
+    if request.node.get_closest_marker("gpu") is None:
+        yield
+        return
+    import faulthandler
+    faulthandler.dump_traceback_later(100, repeat=False)
+    try:
+        yield
+    finally:
+        faulthandler.cancel_dump_traceback_later()
+
+
 @pytest.fixture(scope="session")
 def gpu():
     if not have_gpu():

@@ -1,0 +1,101 @@
+"""GPU: classes with an f shell on the workgroup-per-quartet kernels.
+
+* config-2 sample ((dd|dd), (fd|fd), (ff|dd), (ff|ff), K 1..3, exponents
+  10^U[log 0.05, log 50)) against the oracle in both Boys modes, to the FP64
+  parity bar;
+* every workgroup class (f shells, each canonical class in one orientation and
+  its mirror images), Cartesian and spherical;
+* the FP32-recurrence variant: max absolute and relative error against FP64
+  stated and bounded;
+* routed execution over workgroup bins: FP64 / FP32 / skip codes give exactly
+  the FP64 block, the FP32 block and zeros.
+"""
+import numpy as np
+import pytest
+
+from conftest import assert_parity
+from intception_amd import basis, codegen, runtime
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def _oracle(batch, mode=0):
+    oracle.set_boys_mode(oracle.BOYS_EXACT if mode == 0 else oracle.BOYS_REFERENCE)
+    return oracle.eri_batch(batch, batch.spherical, basis.SPH)
+
+
+def _gpu(batch, mode=0):
+    runtime.set_boys_mode(mode)
+    try:
+        return runtime.compute(batch)
+    finally:
+        runtime.set_boys_mode(0)
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_config2_sample(gpu, mode):
+    b = basis.config2(n_quartet=160, seed=202602)
+    print("config2 sample: %d quartets, boys mode %d" % (b.n_quartet, mode), flush=True)
+    assert_parity(_gpu(b, mode), _oracle(b, mode), what="config2 mode %d" % mode)
+
+
+@pytest.mark.parametrize("sph", [False, True])
+def test_every_workgroup_class(gpu, sph):
+    import itertools
+    covered = codegen.class_set(2, ())
+    rows = []
+    for c in codegen.wg_classes(3, covered):
+        la, lb, lc, ld = c
+        # canonical orientation and its swapped images (ba|dc), (cd|ab)
+        rows += [(la, lb, lc, ld), (lb, la, ld, lc), (lc, ld, la, lb)]
+    cls = np.array(rows, dtype=np.int32)
+    b = basis.make_random_quartets(9100 + sph, cls, [1, 2], sph, log_alpha=(np.log10(0.05), np.log10(50.0)))
+    print("workgroup classes: %d quartets, spherical=%s" % (b.n_quartet, sph), flush=True)
+    assert_parity(_gpu(b), _oracle(b), what="workgroup classes sph=%s" % sph)
+    del itertools
+
+
+def test_config2_fp32_recurrences(gpu):
+    """FP32 recurrences on config 2, per class: error of each quartet relative
+    to its block maximum.  Measured on MI355X (400 quartets, seed 202602):
+    median 1e-6 ((dd|dd)) to 1e-5 ((ff|ff)), p90 up to 3e-4, max up to 4e-3 --
+    FP32 keeps about seven digits of the recurrence intermediates, and the HRR
+    transfer cancels some of them -- which is why the router sends a quartet to
+    this path only below calibrated magnitude and Schwarz thresholds."""
+    import torch
+    from intception_amd.codegen import canonical_np
+    b = basis.config2(n_quartet=400, seed=202602)
+    plan = runtime.EriPlan(b)
+    ref = plan.run().cpu().numpy()
+    got = plan.run_routed(torch.ones(b.n_quartet, dtype=torch.int8, device=plan.device)).cpu().numpy()
+    torch.cuda.synchronize()
+    assert np.all(np.isfinite(got))
+    err = np.abs(got - ref)
+    bm = np.maximum.reduceat(np.abs(ref), b.out_off)
+    rel = np.maximum.reduceat(err, b.out_off) / np.maximum(bm, 1e-300)
+    can = canonical_np(b.classes())
+    print("config2 FP32 recurrences: max abs err %.3e" % err.max(), flush=True)
+    for c in sorted(set(map(tuple, can))):
+        sel = np.all(can == np.array(c), axis=1)
+        r = rel[sel]
+        print("  class %s: err/blockmax median %.1e p90 %.1e max %.1e" % (c, np.median(r), np.percentile(r, 90),
+                                                                        r.max()), flush=True)
+        assert np.median(r) < 1e-4
+        assert np.percentile(r, 90) < 1e-3
+
+
+def test_routed_workgroup_bins(gpu):
+    import torch
+    b = basis.config2(n_quartet=300, seed=31)
+    plan = runtime.EriPlan(b)
+    ref64 = plan.run().cpu().numpy()
+    all32 = plan.run_routed(torch.ones(b.n_quartet, dtype=torch.int8, device=plan.device)).cpu().numpy()
+    codes = np.random.default_rng(2).integers(0, 3, size=b.n_quartet).astype(np.int8)
+    out = torch.full((b.out_size,), 5.0, dtype=torch.float64, device=plan.device)
+    got = plan.run_routed(torch.from_numpy(codes).to(plan.device), out).cpu().numpy()
+    sz = b.sizes()
+    for q in range(b.n_quartet):
+        o, n = b.out_off[q], sz[q]
+        want = ref64[o:o + n] if codes[q] == 0 else (all32[o:o + n] if codes[q] == 1 else np.zeros(n))
+        assert np.array_equal(got[o:o + n], want), q
