@@ -30,24 +30,37 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(batch, n_sample, seed=7):
+def cpu_baseline(batch, n_sample, seed=7, target_s=12.0, chunk=2_000_000):
     """The oracle (C restatement of the reference's generated code, OpenMP)
-    on a random sample of the same workload, on this host's cores."""
+    on random samples of the same workload, on this host's cores: a first
+    sample of n_sample quartets sets the rate, then samples of at most `chunk`
+    quartets run until about target_s seconds of CPU work are timed."""
     from intception_amd import basis
     from oracle import oracle
     rng = np.random.default_rng(seed)
-    idx = np.sort(rng.choice(batch.n_quartet, size=min(n_sample, batch.n_quartet), replace=False))
-    sub = batch.subset(idx)
     threads = min(16, os.cpu_count() or 1)
     oracle.set_boys_mode(oracle.BOYS_REFERENCE)
-    oracle.eri_batch(sub.subset(np.arange(min(500, sub.n_quartet))), sub.spherical, basis.SPH, threads)
-    t0 = time.perf_counter()
-    oracle.eri_batch(sub, sub.spherical, basis.SPH, threads)
-    dt = time.perf_counter() - t0
-    return {"value": float(sub.out_size / dt), "unit": "integrals/s", "cores": threads, "kind": "port",
-            "sample": "%d random quartets of the same workload (%.1f s); oracle/eri_oracle.c, gcc -O3 "
-                      "-march=x86-64-v3 -fopenmp, schedule(dynamic,64)" % (sub.n_quartet, dt),
-            "quartets_per_s": float(sub.n_quartet / dt)}
+    warm = batch.subset(np.sort(rng.choice(batch.n_quartet, size=min(500, batch.n_quartet), replace=False)))
+    oracle.eri_batch(warm, warm.spherical, basis.SPH, threads)
+    n_q, n_i, dt = 0, 0, 0.0
+    n = min(n_sample, batch.n_quartet)
+    while True:
+        sub = batch.subset(np.sort(rng.choice(batch.n_quartet, size=n, replace=False)))
+        t0 = time.perf_counter()
+        oracle.eri_batch(sub, sub.spherical, basis.SPH, threads)
+        dt += time.perf_counter() - t0
+        n_q += sub.n_quartet
+        n_i += sub.out_size
+        del sub
+        if dt >= target_s or n_q >= 4 * batch.n_quartet:
+            break
+        rate = n_q / dt
+        n = int(min(chunk, batch.n_quartet, max(1000, rate * (target_s - dt) * 1.05)))
+    return {"value": float(n_i / dt), "unit": "integrals/s", "cores": threads, "kind": "port",
+            "sample": "%d quartets drawn at random from the same workload in samples of <= %d (%.1f s of CPU "
+                      "work); oracle/eri_oracle.c, gcc -O3 -march=x86-64-v3 -fopenmp, schedule(dynamic,64), "
+                      "reference Boys scheme" % (n_q, max(chunk, n_sample), dt),
+            "quartets_per_s": float(n_q / dt)}
 
 
 def _timed(torch, dev, stream, fn, n):

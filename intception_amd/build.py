@@ -24,6 +24,10 @@ FLAGS = ["-O3", "-fPIC", "-std=c++17", "--offload-arch=" + ARCH,
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
          "-Wno-unused-command-line-argument"]
 
+# occupancy targets (waves per SIMD) for class kernels that reach them without
+# spilling: the compiler otherwise settles at one wave fewer
+OCCUPANCY = {}
+
 # classes compiled into the library: every canonical class with shells up to
 # l = 2 (Cartesian and spherical), plus the three-centre set (ab|P s) with
 # a, b <= d and auxiliary P up to g (the unit s-shell has exponent 0)
@@ -62,7 +66,7 @@ def build(lmax=2, jobs=None, verbose=True):
         boys_tables.write_inc(inc)
     classes = codegen.class_set(lmax, EXTRA_CLASSES)
     wg = codegen.wg_classes(3, classes)
-    files, metas = codegen.generate(GEN, classes, wg=wg)
+    files, metas = codegen.generate(GEN, classes, wg=wg, occupancy=OCCUPANCY)
     common = [os.path.join(CSRC, "intc_common.h")]
     srcs = [(f, common) for f in files]
     srcs.append((os.path.join(CSRC, "intc_runtime.hip"),

@@ -192,7 +192,7 @@ def _hrr_lines(src, lo, nmove, shift_vec, prefix, kind):
     return L, (lambda a_, b_: names[(a_, b_)])
 
 
-def emit_class_source(cls, spherical, prec="d"):
+def emit_class_source(cls, spherical, prec="d", waves_per_eu=None):
     """prec "d": FP64 throughout.  prec "f": the per-primitive recurrences (bra
     chain, ket VRR, cross-lane terms) in FP32 on the Boys values F_m; FP64 for
     the Boys function, the prefactor exp(-mu R^2) terms, the contraction sums
@@ -423,8 +423,11 @@ def emit_class_source(cls, spherical, prec="d"):
     o.ind = 0
     o("}")
     o("")
-    o('extern "C" __global__ void __launch_bounds__(256) %s(const IntcEriArgs a) { %s_body<0>(a); }'
-      % (name, name))
+    # waves_per_eu: occupancy target for the Chebyshev-mode kernel (the compiler
+    # then caps its registers at 512 / waves VGPRs)
+    occ = " __attribute__((amdgpu_waves_per_eu(%d)))" % waves_per_eu if waves_per_eu else ""
+    o('extern "C" __global__ void __launch_bounds__(256)%s %s(const IntcEriArgs a) { %s_body<0>(a); }'
+      % (occ, name, name))
     o('extern "C" __global__ void __launch_bounds__(256) %s_refboys(const IntcEriArgs a) { %s_body<1>(a); }'
       % (name, name))
     o('extern "C" hipError_t %s_launch(const IntcEriArgs *a, int grid, int block, size_t lds, hipStream_t st)'
@@ -464,13 +467,14 @@ def _nout(l, sph):
     return 2 * l + 1 if sph else (l + 1) * (l + 2) // 2
 
 
-def generate(outdir, classes, sph_modes=(False, True), precs=("d", "f"), wg=()):
+def generate(outdir, classes, sph_modes=(False, True), precs=("d", "f"), wg=(), occupancy=None):
     os.makedirs(outdir, exist_ok=True)
     metas = []
     written = []
     for cls in classes:
         for sph, prec in itertools.product(sph_modes, precs):
-            src, meta = emit_class_source(cls, sph, prec)
+            nm = kernel_name(cls, sph, prec)
+            src, meta = emit_class_source(cls, sph, prec, (occupancy or {}).get(nm))
             fn = os.path.join(outdir, meta["name"] + ".hip")
             old = open(fn).read() if os.path.exists(fn) else None
             if old != src:
