@@ -1,15 +1,23 @@
 """Benchmark: contracted ERIs/s on one MI355X.
 
-Workload (default): 1,000,000 contracted shell quartets, class drawn uniformly
-over (la lb|lc ld) with l in {s,p,d} (81 classes), contraction depth per shell
-uniform in 1..6, centres uniform in a 10-bohr cube, exponents log-uniform in
-[0.1, 100], spherical output, FP64.  A step is one pass of the whole batch:
-device pair builder + one kernel per canonical class bin, inputs resident in
-HBM.  Host-side planning (class binning) happens once, before the timed region.
+Default workload (config 3): every Schwarz-screened (1e-12) unique shell
+quartet of a 200-atom molecule (shells s,s,p,p,d per atom, 68.5 M quartets,
+4.17e9 integrals, spherical).  Two legs are timed on the same quartet list:
+plain FP64 (pair builder + every class bin in one hipGraph) and the learned
+path (router kernel: features + MFMA network + route codes, then the routed
+graph: route compaction, skipped-block zeroing, FP64 and FP32-recurrence
+queues).  `value` is the whole-job integrals/s of the faster learned variant
+that keeps max |error| <= 1e-10 against the plain FP64 result; the plain leg
+is reported beside it.  A step is one pass over the whole quartet list with
+inputs resident in HBM; router training, screening and planning happen once,
+before the timed region.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu]
+                    [--workload config3|config4|config1]
 
-Prints one JSON line.
+config4: the (ab|P s) three-centre workload of a 100-atom molecule with an
+auxiliary basis (same two legs).  config1: 1 M random quartets over the 81
+(s,p,d) classes, K 1..6, FP64 only.  Prints one JSON line.
 """
 import argparse
 import json
@@ -85,26 +93,33 @@ def _per_bin(torch, plan, out, flops_q, canon, stream, reps=3, prec=0):
     return res
 
 
-def run_config3(args):
+def run_molecule(args, three_center=False):
     """Molecule-like screened workload: plain FP64 vs learned screening +
-    FP32/FP64 routing over every Schwarz-screened quartet of the molecule."""
+    FP32/FP64 routing over every Schwarz-screened quartet of the molecule
+    (config 3), or over every screened (ab|P s) three-centre quartet of a
+    molecule with an auxiliary basis (config 4)."""
     import torch
     from intception_amd import autotune, basis, flops_model, lib, runtime, screening
     from intception_amd.codegen import canonical_np
     lib.check(lib.load().intc_init())
     t0 = time.perf_counter()
-    router, rstats = screening.build_router(n_train=args.train, n_atoms=args.atoms, epochs=15, n_val=1_000_000,
-                                            log=log)
+    n_atoms = args.atoms or (100 if three_center else 200)
+    mol_seed = 202604 if three_center else 202603
+    train_seeds, val_seed = ((202621, 202622), 202629) if three_center else ((202611, 202612), 202619)
+    router, rstats = screening.build_router(n_train=args.train, n_atoms=n_atoms, epochs=15, n_val=1_000_000,
+                                            log=log, three_center=three_center, train_seeds=train_seeds,
+                                            val_seed=val_seed)
     t_router = time.perf_counter() - t0
     t0 = time.perf_counter()
-    mol, pf = screening.molecule_setup(args.atoms, 202603)
+    mol, pf = screening.molecule_setup(n_atoms, mol_seed, three_center=three_center)
     # quartets listed in the plan's order (class, primitive count): route codes
     # and skipped output blocks are then read and written sequentially
     pp = screening.plan_sorted(mol, mol.screened_quartets(1e-12))
     batch = mol.pair_batch(pp)
     t_setup = time.perf_counter() - t0
-    log("config3 molecule: %d atoms, %d shells, %d pairs, %d screened quartets, %d integrals (%.1fs)"
-        % (args.atoms, mol.n_shell, len(mol.pairs), batch.n_quartet, batch.out_size, t_setup))
+    log("%s molecule: %d atoms, %d shells, %d pairs, %d screened quartets, %d integrals (%.1fs)"
+        % ("config4" if three_center else "config3", n_atoms, mol.n_shell, len(mol.pairs), batch.n_quartet,
+           batch.out_size, t_setup))
     t0 = time.perf_counter()
     plan = runtime.EriPlan(batch)
     t_plan = time.perf_counter() - t0
@@ -196,12 +211,16 @@ def run_config3(args):
         "vs_baseline": None,
         "dtype": "f64+f32 (learned routing)",
         "data": "synthetic",
-        "config": {"workload": "config3: %d-atom molecule (random sequential packing, r = %.1f bohr, spacing "
-                               ">= 1.8), shells s,s,p,p,d K=4,1,3,1,1 even-tempered; all %d unique shell quartets "
-                               "passing Schwarz 1e-12; spherical" % (args.atoms, mol.radius, batch.n_quartet),
-                   "quartets": int(batch.n_quartet), "integrals": int(n_int), "seed": 202603,
-                   "router_training": "%d quartets from molecules seeded 202611/202612, calibrated on 202619"
-                                      % rstats["train_samples"]},
+        "config": {"workload": (("config4: %d-atom molecule (random sequential packing, r = %.1f bohr, spacing "
+                                 ">= 1.8), orbital s,s,p,p,d K~U{1..6}, 12 auxiliary shells/atom l<=4 K=1; all %d "
+                                 "(ab|P s) quartets passing Schwarz 1e-12; spherical")
+                                if three_center else
+                                ("config3: %d-atom molecule (random sequential packing, r = %.1f bohr, spacing "
+                                 ">= 1.8), shells s,s,p,p,d K=4,1,3,1,1 even-tempered; all %d unique shell quartets "
+                                 "passing Schwarz 1e-12; spherical")) % (n_atoms, mol.radius, batch.n_quartet),
+                   "quartets": int(batch.n_quartet), "integrals": int(n_int), "seed": mol_seed,
+                   "router_training": "%d quartets from molecules seeded %d/%d, calibrated on %d"
+                                      % (rstats["train_samples"], train_seeds[0], train_seeds[1], val_seed)},
         "plain_fp64": {"ms_per_step": float(ms64), "integrals_per_s": float(n_int / (ms64 * 1e-3)),
                        "quartets_per_s": float(batch.n_quartet / (ms64 * 1e-3))},
         "learned_skip_fp64": learned_skip,
@@ -223,6 +242,14 @@ def run_config3(args):
     }
     plan.close()
     return res
+
+
+def run_config3(args):
+    return run_molecule(args, three_center=False)
+
+
+def run_config4(args):
+    return run_molecule(args, three_center=True)
 
 
 def run_config1(args):
@@ -371,9 +398,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", choices=["config3", "config1"], default="config3")
+    ap.add_argument("--workload", choices=["config3", "config1", "config4"], default="config3")
     ap.add_argument("--quartets", type=int, default=1_000_000, help="config1 size")
-    ap.add_argument("--atoms", type=int, default=200, help="config3 molecule size")
+    ap.add_argument("--atoms", type=int, default=None, help="molecule size (config3 200, config4 100)")
     ap.add_argument("--train", type=int, default=2_000_000, help="config3 training quartets")
     ap.add_argument("--cpu-sample", type=int, default=150_000)
     ap.add_argument("--no-cpu", action="store_true")
@@ -386,7 +413,7 @@ def main():
         print("bench.py measures one GPU only: quartets shard across GPUs without exchange "
               "(see the sharding design); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
-    res = run_config3(args) if args.workload == "config3" else run_config1(args)
+    res = {"config3": run_config3, "config4": run_config4, "config1": run_config1}[args.workload](args)
     print(json.dumps(res), flush=True)
 
 

@@ -455,12 +455,19 @@ def class_set(lmax_pair_side=2, extra=()):
     return sorted(out)
 
 
-def wg_classes(lmax=3, covered=()):
+def wg_classes(lmax=3, covered=(), extra=((4, 0, 4, 0),)):
     """Canonical 4-centre classes with shells up to lmax that have no register
     kernel (an f shell makes the bra group or the per-lane ket table too
-    large): they run on the workgroup-per-quartet kernels."""
+    large): they run on the workgroup-per-quartet kernels.  extra: further
+    classes for those kernels, by default (g s|g s), the Schwarz diagonal of
+    a g auxiliary shell with the unit s-shell."""
     cov = set(covered)
-    return [c for c in canonical_classes(lmax) if c not in cov]
+    out = [c for c in canonical_classes(lmax) if c not in cov]
+    for c in extra:
+        c = canonical(*c)[0]
+        if c not in cov and c not in out:
+            out.append(c)
+    return out
 
 
 def _nout(l, sph):

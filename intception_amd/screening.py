@@ -22,8 +22,10 @@ import numpy as np
 from . import basis, mlp, molecule, runtime
 
 
-def molecule_setup(n_atoms, seed, tau=1e-12):
-    mol = molecule.Molecule(n_atoms, seed=seed)
+def molecule_setup(n_atoms, seed, tau=1e-12, three_center=False):
+    """Molecule (config 3) or three-centre molecule (config 4) with Schwarz
+    factors and the per-pair feature table."""
+    mol = (molecule.ThreeCenterMolecule if three_center else molecule.Molecule)(n_atoms, seed=seed)
     mol.schwarz()
     pf = mlp.pair_features(mol.shell_l, mol.shell_nprim, mol.shell_off, mol.shell_cen, mol.exps, mol.pairs, mol.Q)
     return mol, pf
@@ -49,7 +51,7 @@ def sample_targets(mol, n, seed, tau=1e-12, fp32=False):
     """Uniform sample of the screened quartets with FP64 targets (and the
     FP32-recurrence error per quartet when fp32=True)."""
     import torch
-    pp, total = molecule.sample_screened(mol.Q, tau, n, seed=seed)
+    pp, total = mol.sample_screened(tau, n, seed=seed)
     b = mol.pair_batch(pp)
     plan = runtime.EriPlan(b)
     out = plan.run()
@@ -66,12 +68,12 @@ def sample_targets(mol, n, seed, tau=1e-12, fp32=False):
     return pp, target, e32, total
 
 
-def training_set(n_samples, seeds, n_atoms=200, tau=1e-12, log=None):
+def training_set(n_samples, seeds, n_atoms=200, tau=1e-12, log=None, three_center=False):
     """Features and targets log10(max|integral| / (Q_P Q_P')) (clamped to
     [-25, 0]) for quartets of molecules with the given seeds."""
     feats, targets = [], []
     for s in seeds:
-        mol, pf = molecule_setup(n_atoms, s, tau)
+        mol, pf = molecule_setup(n_atoms, s, tau, three_center)
         pp, t, _, total = sample_targets(mol, n_samples // len(seeds), seed=s + 7, tau=tau)
         F = mlp.quartet_features_np(pf, pp).astype(np.float32)
         feats.append(F)
@@ -120,12 +122,14 @@ class LearnedRouter:
 
 
 def build_router(n_train=2_000_000, train_seeds=(202611, 202612), val_seed=202619, n_val=400_000, n_atoms=200,
-                 tau=1e-12, tau_skip=5e-11, fp32_budget=1e-11, epochs=10, under_weight=10.0, log=None):
-    """Train and calibrate a router on molecules of their own seeds."""
+                 tau=1e-12, tau_skip=5e-11, fp32_budget=1e-11, epochs=10, under_weight=10.0, log=None,
+                 three_center=False):
+    """Train and calibrate a router on molecules of their own seeds
+    (three-centre molecules for the (ab|P s) workload)."""
     import time
     import torch
     t0 = time.perf_counter()
-    X, T = training_set(n_train, train_seeds, n_atoms, tau, log)
+    X, T = training_set(n_train, train_seeds, n_atoms, tau, log, three_center)
     t_data = time.perf_counter() - t0
     net = mlp.MagnitudeNet()
     mean, scale = X.mean(0), X.std(0) + 1e-6
@@ -138,7 +142,7 @@ def build_router(n_train=2_000_000, train_seeds=(202611, 202612), val_seed=20261
     torch.cuda.synchronize()
     t_train = time.perf_counter() - t0
     # calibration on a held-out molecule
-    mol, pf = molecule_setup(n_atoms, val_seed, tau)
+    mol, pf = molecule_setup(n_atoms, val_seed, tau, three_center)
     pp, tv, e32, total = sample_targets(mol, n_val, seed=val_seed + 7, tau=tau, fp32=True)
     d_pf, d_qp = _device(pf, torch), _device(pp.astype(np.int32), torch)
     pred = net.predict(d_pf, d_qp).cpu().numpy()

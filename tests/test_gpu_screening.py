@@ -35,3 +35,45 @@ def test_learned_routing_small_molecule(gpu):
           flush=True)
     assert err <= 1e-10
     assert codes[2] + codes[1] > 0.05
+
+
+def test_three_center_molecule(gpu):
+    """Config-4 style: (ab|P s) quartets of a small molecule with an auxiliary
+    basis.  FP64 parity of a sample against the oracle (bar as for the
+    three-centre classes: diffuse auxiliary shells far apart lose up to ~1e-11
+    of the block maximum to rounding in the oracle's own scheme), then the
+    learned router (trained on three-centre molecules of their own seeds) keeps
+    the routed result within 1e-10 of FP64."""
+    import torch
+    from conftest import assert_parity  # noqa: F401
+    from intception_amd import basis
+    from oracle import oracle
+    mol, pf = screening.molecule_setup(10, 41, three_center=True)
+    pp = mol.screened_quartets(1e-12)
+    assert len(pp) > 0
+    samp = pp[np.random.default_rng(0).choice(len(pp), size=min(1500, len(pp)), replace=False)]
+    b = mol.pair_batch(samp)
+    got = runtime.compute(b)
+    oracle.set_boys_mode(oracle.BOYS_EXACT)
+    ref = oracle.eri_batch(b, True, basis.SPH)
+    sz = b.sizes()
+    for q in range(b.n_quartet):
+        o, n = b.out_off[q], sz[q]
+        g, r = got[o:o + n], ref[o:o + n]
+        lim = 1e-14 + 1e-12 * np.abs(r) + 5e-11 * np.abs(r).max()
+        assert np.all(np.abs(g - r) <= lim), (q, np.abs(g - r).max(), np.abs(r).max())
+    router, stats = screening.build_router(n_train=150_000, train_seeds=(51, 52), val_seed=59, n_val=50_000,
+                                           n_atoms=10, epochs=6, three_center=True)
+    print("3c router stats:", {k: v for k, v in stats.items() if k != "train_mse"}, flush=True)
+    pp = screening.plan_sorted(mol, pp)
+    b = mol.pair_batch(pp)
+    plan = runtime.EriPlan(b)
+    ref = plan.run()
+    route = torch.empty(b.n_quartet, dtype=torch.int8, device="cuda")
+    router.route(torch.from_numpy(pf).cuda(), torch.from_numpy(pp.astype(np.int32)).cuda(), route)
+    got = plan.run_routed(route)
+    torch.cuda.synchronize()
+    err = (got - ref).abs().max().item()
+    codes = np.bincount(route.cpu().numpy().astype(np.int64), minlength=3) / b.n_quartet
+    print("3c screened quartets %d; fp64/fp32/skip = %s; max abs err %.3e" % (b.n_quartet, codes, err), flush=True)
+    assert err <= 1e-10
