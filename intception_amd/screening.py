@@ -141,19 +141,27 @@ def build_router(n_train=2_000_000, train_seeds=(202611, 202612), val_seed=20261
                    under_weight=under_weight)
     torch.cuda.synchronize()
     t_train = time.perf_counter() - t0
-    # calibration on a held-out molecule
-    mol, pf = molecule_setup(n_atoms, val_seed, tau, three_center)
-    pp, tv, e32, total = sample_targets(mol, n_val, seed=val_seed + 7, tau=tau, fp32=True)
-    d_pf, d_qp = _device(pf, torch), _device(pp.astype(np.int32), torch)
-    pred = net.predict(d_pf, d_qp).cpu().numpy()
+    # calibration on held-out molecules (val_seed: one seed or several; their
+    # samples are pooled, n_val quartets from each)
+    seeds = tuple(val_seed) if isinstance(val_seed, (tuple, list)) else (val_seed,)
+    preds, tvs, e32s, logqqs = [], [], [], []
+    for vs in seeds:
+        mol, pf = molecule_setup(n_atoms, vs, tau, three_center)
+        pp, tv, e32, total = sample_targets(mol, n_val, seed=vs + 7, tau=tau, fp32=True)
+        d_pf, d_qp = _device(pf, torch), _device(pp.astype(np.int32), torch)
+        preds.append(net.predict(d_pf, d_qp).cpu().numpy())
+        tvs.append(tv)
+        e32s.append(e32)
+        logqqs.append(np.log10(np.maximum(mol.Q[pp[:, 0]] * mol.Q[pp[:, 1]], 1e-300)))
+    pred, tv, e32, logqq = (np.concatenate(x) for x in (preds, tvs, e32s, logqqs))
     margin = net.calibrate(pred, tv, np.log10(tau_skip))
     router = LearnedRouter(net, np.log10(tau_skip), fp32_budget)
-    logqq = np.log10(np.maximum(mol.Q[pp[:, 0]] * mol.Q[pp[:, 1]], 1e-300))
     n32, _, _ = router.calibrate_fp32(pred, logqq, e32)
     stats = {"train_samples": int(len(X)), "train_seconds": t_train, "data_seconds": t_data,
              "train_mse": hist, "val_rmse": float(np.sqrt(np.mean((pred - tv) ** 2))),
              "val_max_underprediction": float(np.max(tv - pred)), "margin": margin,
-             "val_fp32_max_err": float(e32.max()), "val_fp32_fraction": n32 / len(pp),
+             "val_fp32_max_err": float(e32.max()), "val_fp32_fraction": n32 / len(pred),
+             "val_samples": int(len(pred)),
              "val_skip_fraction": float(np.mean(pred + margin < np.log10(tau_skip))), **router.summary()}
     if log:
         log("router: %s" % {k: v for k, v in stats.items() if k != "train_mse"})
