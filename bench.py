@@ -349,6 +349,7 @@ def run_config1(args):
         per_bin.append(dict(b, ms=ms, flops=float(flops_q[sel].sum()), integrals=int(batch.sizes()[sel].sum())))
     dom = max(per_bin, key=lambda r: r["ms"])
     achieved = dom["flops"] / (dom["ms"] * 1e-3) / 1e12
+    dom_name = "intc_eri_d%d%d%d%d_s" % tuple(dom["cls"])
     total_flops = float(flops_q.sum())
 
     # parity spot check against the oracle (Chebyshev Boys vs accurate Boys)
@@ -395,7 +396,7 @@ def run_config1(args):
         "quartets_per_s": float(batch.n_quartet / (ms_step * 1e-3)),
         "roofline": {"bound": "mfma", "achieved": float(achieved), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": float(achieved / FP64_PEAK_TFLOPS),
-                     "traffic": measured_traffic(dom_name, "config4" if three_center else "config3"),
+                     "traffic": measured_traffic(dom_name, "config1"),
                      "traffic_unit": "bytes per launch (HBM, rocprofv3 FETCH_SIZE + WRITE_SIZE)",
                      "kernel": dom_name, "kernel_ms": dom["ms"],
                      "kernel_flops": dom["flops"],

@@ -111,6 +111,14 @@ class Emitter:
         return "\n".join(self.lines)
 
 
+# groups of at least this many lanes whose ket reaches lc + ld in BATCH_LK use
+# the batched setup loop (measured on MI355X, config-3 bins: 1.06-1.37x for
+# ket 2-3; for a one-level ket the recurrences are too short to pay for the
+# LDS round trip, and the ket-4 kernels sit at the register limit)
+BATCH_MIN_G = 4
+BATCH_LK = (2, 3)
+
+
 def kernel_name(cls, spherical, prec="d"):
     return "intc_eri_%s%d%d%d%d_%s" % (prec, *cls, "s" if spherical else "c")
 
@@ -246,14 +254,35 @@ def emit_class_source(cls, spherical, prec="d", waves_per_eu=None):
     o("__device__ __forceinline__ void %s_body(const IntcEriArgs &a)" % name)
     o("{")
     o.ind = 1
-    o("constexpr int G = %d, NE = %d, NCD = %d, M = %d;" % (G, NE, NCD, M))
+    # Batched setup (groups of G >= BATCH_MIN_G lanes): lane e of a group
+    # computes the Boys values and recurrence coefficients of the group's
+    # primitive quartet kb + e into its LDS slot, then the group walks the G
+    # slots.  The setup (pair loads, Boys, exp, divisions) is then issued once
+    # per G primitive quartets instead of once per quartet in every lane.
+    batched = G >= BATCH_MIN_G and LR in BATCH_LK
+    slot = []          # (name, C type in the recurrences) in slot order
+    for m in range(M + 1):
+        slot.append(("c0_%d" % m, R))
+    if LX > 0:
+        slot += [(n, R) for n in ("PAx", "PAy", "PAz", "WPx", "WPy", "WPz", "roz")] + [("o2z", "double")]
+    if LR > 0:
+        slot += [(n, R) for n in ("QCx", "QCy", "QCz", "WQx", "WQy", "WQz", "roe")] + [("o2e", "double")]
+        if LX > 0:
+            slot.append(("o2s", R))
+    if prec == "f":
+        slot.append(("pref", "double"))
+    NS = len(slot) if batched else 0
+    o("constexpr int G = %d, NE = %d, NCD = %d, M = %d, NS = %d;" % (G, NE, NCD, M, NS))
     o("extern __shared__ double lds[];")
     o("const int T = a.team_groups, TL = G * T, TPW = 64 / TL;")
     o("const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;")
     o("const int tw = lane / TL, tl = lane - tw * TL;")
     o("const int grp = tl / G, e = tl - grp * G;")
     o("if (tw >= TPW) return;")
-    o("double *tlds = lds + (size_t)(wave * TPW + tw) * (NE * NCD);")
+    o("// per-wave LDS region: the setup slots (batched loop), then the epilogue transpose")
+    o("const int REG = (TPW * NE * NCD > 64 * NS) ? TPW * NE * NCD : 64 * NS;")
+    o("double *wlds = lds + (size_t)wave * REG;")
+    o("double *tlds = wlds + (size_t)tw * (NE * NCD);")
     o("const int *lt = %s_lane[e];" % name)
     o("const %s ex_d = (%s)lt[0], ey_d = (%s)lt[1], ez_d = (%s)lt[2];" % (R, R, R, R))
     o("const int lbase = lane - e;")
@@ -276,70 +305,121 @@ def emit_class_source(cls, spherical, prec="d", waves_per_eu=None):
     acc_f = [f for f in ket_comps if sum(f) >= LC]
     for f in acc_f:
         o("double acc%d = 0.0;" % cidx(f))
-    o("int ib = grp / Pk, ik = grp - ib * Pk;")
-    o("for (int p = grp; p < P; p += T) {")
-    o.ind = 3
-    o("const double zeta = a.pp_zeta[ob + ib], o2z = a.pp_o2z[ob + ib];")
-    o("const double Px = a.pp_px[ob + ib], Py = a.pp_py[ob + ib], Pz = a.pp_pz[ob + ib];")
-    o("const double kab = a.pp_kappa[ob + ib];")
-    o("const double eta = a.pp_zeta[okk + ik], o2e = a.pp_o2z[okk + ik];")
-    o("const double Qx = a.pp_px[okk + ik], Qy = a.pp_py[okk + ik], Qz = a.pp_pz[okk + ik];")
-    o("const double kcd = a.pp_kappa[okk + ik];")
-    o("(void)o2z; (void)o2e;")
-    o("ik += T; while (ik >= Pk) { ik -= Pk; ++ib; }")
-    o("const double os = 1.0 / (zeta + eta);")
-    o("const double rho = zeta * eta * os;")
-    o("const double PQx = Px - Qx, PQy = Py - Qy, PQz = Pz - Qz;")
-    o("const double Tb = rho * (PQx * PQx + PQy * PQy + PQz * PQz);")
-    o("const double pref = kab * kcd * sqrt(os);")
-    o("double F[M + 1];")
-    o("intc_boys<M, BM>(Tb, F, a);")
-    if prec == "f":
-        # FP32 recurrences on s^m F_m with s = max(1, 2T): F_m(T) ~ (2m-1)!! /
-        # (2T)^m F_0 underflows FP32 for large T, s^m F_m stays O(F_0).  Every
-        # term that raises m by one carries 1/s; results at m = 0 are unscaled.
-        # The prefactor multiplies in FP64 at accumulation.
-        o("const float o2zf = (float)o2z, o2ef = (float)o2e;")
-        o("const double sc = Tb > 0.5 ? 2.0 * Tb : 1.0, isc = 1.0 / sc;")
-    else:
-        o("const double isc = 1.0;")
-    if LX > 0:
-        o("const double eo = eta * os;")
-        o("const %s PAx = (%s)(Px - Ax), PAy = (%s)(Py - Ay), PAz = (%s)(Pz - Az);" % (R, R, R, R))
-        o("const %s WPx = (%s)(-eo * PQx * isc), WPy = (%s)(-eo * PQy * isc), WPz = (%s)(-eo * PQz * isc);"
-          % (R, R, R, R))
-        o("const %s roz = (%s)(eo * isc);" % (R, R))
-    if LR > 0:
-        o("const double zo = zeta * os;")
-        o("const %s QCx = (%s)(Qx - Cx), QCy = (%s)(Qy - Cy), QCz = (%s)(Qz - Cz);" % (R, R, R, R))
-        o("const %s WQx = (%s)(zo * PQx * isc), WQy = (%s)(zo * PQy * isc), WQz = (%s)(zo * PQz * isc);"
-          % (R, R, R, R))
-        o("const %s roe = (%s)(zo * isc);" % (R, R))
+    def emit_setup(load_idx):
+        """pair loads + Boys + coefficients for primitive quartet (ib, ik)"""
+        o("const double zeta = a.pp_zeta[ob + ib], o2z = a.pp_o2z[ob + ib];")
+        o("const double Px = a.pp_px[ob + ib], Py = a.pp_py[ob + ib], Pz = a.pp_pz[ob + ib];")
+        o("const double kab = a.pp_kappa[ob + ib];")
+        o("const double eta = a.pp_zeta[okk + ik], o2e = a.pp_o2z[okk + ik];")
+        o("const double Qx = a.pp_px[okk + ik], Qy = a.pp_py[okk + ik], Qz = a.pp_pz[okk + ik];")
+        o("const double kcd = a.pp_kappa[okk + ik];")
+        o("(void)o2z; (void)o2e;")
+        if load_idx:
+            o("ik += T; while (ik >= Pk) { ik -= Pk; ++ib; }")
+        o("const double os = 1.0 / (zeta + eta);")
+        o("const double rho = zeta * eta * os;")
+        o("const double PQx = Px - Qx, PQy = Py - Qy, PQz = Pz - Qz;")
+        o("const double Tb = rho * (PQx * PQx + PQy * PQy + PQz * PQz);")
+        o("const double pref = kab * kcd * sqrt(os);")
+        o("double F[M + 1];")
+        o("intc_boys<M, BM>(Tb, F, a);")
+        if prec == "f":
+            # FP32 recurrences on s^m F_m with s = max(1, 2T): F_m(T) ~ (2m-1)!! /
+            # (2T)^m F_0 underflows FP32 for large T, s^m F_m stays O(F_0).  Every
+            # term that raises m by one carries 1/s; results at m = 0 are unscaled.
+            # The prefactor multiplies in FP64 at accumulation.
+            o("const float o2zf = (float)o2z, o2ef = (float)o2e;")
+            o("(void)o2zf; (void)o2ef;")
+            o("const double sc = Tb > 0.5 ? 2.0 * Tb : 1.0, isc = 1.0 / sc;")
+        else:
+            o("const double isc = 1.0;")
         if LX > 0:
-            o("const %s o2s = (%s)(0.5 * os * isc);" % (R, R))
+            o("const double eo = eta * os;")
+            o("const %s PAx = (%s)(Px - Ax), PAy = (%s)(Py - Ay), PAz = (%s)(Pz - Az);" % (R, R, R, R))
+            o("const %s WPx = (%s)(-eo * PQx * isc), WPy = (%s)(-eo * PQy * isc), WPz = (%s)(-eo * PQz * isc);"
+              % (R, R, R, R))
+            o("const %s roz = (%s)(eo * isc);" % (R, R))
+        if LR > 0:
+            o("const double zo = zeta * os;")
+            o("const %s QCx = (%s)(Qx - Cx), QCy = (%s)(Qy - Cy), QCz = (%s)(Qz - Cz);" % (R, R, R, R))
+            o("const %s WQx = (%s)(zo * PQx * isc), WQy = (%s)(zo * PQy * isc), WQz = (%s)(zo * PQz * isc);"
+              % (R, R, R, R))
+            o("const %s roe = (%s)(zo * isc);" % (R, R))
+            if LX > 0:
+                o("const %s o2s = (%s)(0.5 * os * isc);" % (R, R))
+        if prec == "f":
+            o("double scm = 1.0;")
+            for m in range(M + 1):
+                o("const float c0_%d = (float)(F[%d] * scm);%s" % (m, m, " scm *= sc;" if m < M else ""))
+        else:
+            for m in range(M + 1):
+                o("const double c0_%d = pref * F[%d];" % (m, m))
+
+    def emit_recurrences():
+        """bra chain + ket VRR + accumulation on the names of the setup"""
+        if LX > 0 and LR > 0:
             o("const %s cplx = ex_d * o2s, cply = ey_d * o2s, cplz = ez_d * o2s;" % R)
-    if prec == "f":
-        o("double scm = 1.0;")
-        for m in range(M + 1):
-            o("const float c0_%d = (float)(F[%d] * scm);%s" % (m, m, " scm *= sc;" if m < M else ""))
-        body = _chain_lines(LX, M, R)
-        lines, V = _ket_lines(LX, LR, R)
-        body += lines
-        body = [ln.replace("o2z", "o2zf").replace("o2e", "o2ef") for ln in body]
-        for ln in body:
-            o(ln)
-        for f in acc_f:
-            o("acc%d = fma(pref, (double)%s, acc%d);" % (cidx(f), V(f, 0), cidx(f)))
+        if prec == "f":
+            body = _chain_lines(LX, M, R)
+            lines, V = _ket_lines(LX, LR, R)
+            body += lines
+            body = [ln.replace("o2z", "o2zf").replace("o2e", "o2ef") for ln in body]
+            for ln in body:
+                o(ln)
+            for f in acc_f:
+                o("acc%d = fma(pref, (double)%s, acc%d);" % (cidx(f), V(f, 0), cidx(f)))
+        else:
+            for ln in _chain_lines(LX, M):
+                o(ln)
+            lines, V = _ket_lines(LX, LR)
+            for ln in lines:
+                o(ln)
+            for f in acc_f:
+                o("acc%d += %s;" % (cidx(f), V(f, 0)))
+
+    if not batched:
+        o("int ib = grp / Pk, ik = grp - ib * Pk;")
+        o("for (int p = grp; p < P; p += T) {")
+        o.ind = 3
+        emit_setup(True)
+        emit_recurrences()
     else:
-        for m in range(M + 1):
-            o("const double c0_%d = pref * F[%d];" % (m, m))
-        for ln in _chain_lines(LX, M):
-            o(ln)
-        lines, V = _ket_lines(LX, LR)
-        for ln in lines:
-            o(ln)
-        for f in acc_f:
-            o("acc%d += %s;" % (cidx(f), V(f, 0)))
+        o("// this group's primitive quartets are p = grp + k T, k < K")
+        o("const int K = P > grp ? (P - grp + T - 1) / T : 0;")
+        o("double *myslot = wlds + (size_t)lane * NS;")
+        o("const double *gslot = wlds + (size_t)(lane - e) * NS;")
+        o("for (int kb = 0; kb < K; kb += G) {")
+        o.ind = 3
+        o("if (kb + e < K) {")
+        o.ind = 4
+        o("const int pq = grp + (kb + e) * T;")
+        o("const int ib = pq / Pk, ik = pq - ib * Pk;")
+        emit_setup(False)
+        for i, (nm, ty) in enumerate(slot):
+            o("myslot[%d] = (double)%s;" % (i, nm))
+        o.ind = 3
+        o("}")
+        o("__builtin_amdgcn_fence(__ATOMIC_SEQ_CST, \"wavefront\");")
+        o("__builtin_amdgcn_wave_barrier();")
+        o("const int ns = (K - kb < G) ? K - kb : G;")
+        o("for (int st = 0; st < ns; ++st) {")
+        o.ind = 4
+        o("const double *u = gslot + st * NS;")
+        for i, (nm, ty) in enumerate(slot):
+            if ty == "double":
+                o("const double %s = u[%d];" % (nm, i))
+            else:
+                o("const %s %s = (%s)u[%d];" % (ty, nm, ty, i))
+        if prec == "f":
+            if LX > 0:
+                o("const float o2zf = (float)o2z;")
+            if LR > 0:
+                o("const float o2ef = (float)o2e;")
+        emit_recurrences()
+        o.ind = 3
+        o("}")
+        o("__builtin_amdgcn_fence(__ATOMIC_SEQ_CST, \"wavefront\");")
+        o("__builtin_amdgcn_wave_barrier();")
     o.ind = 2
     o("}")
     # team reduction over groups (tree towards group 0)
@@ -433,6 +513,11 @@ def emit_class_source(cls, spherical, prec="d", waves_per_eu=None):
     o('extern "C" hipError_t %s_launch(const IntcEriArgs *a, int grid, int block, size_t lds, hipStream_t st)'
       % name)
     o("{")
+    o("    // per wave: max(epilogue transpose of its teams, 64 setup slots)")
+    o("    (void)lds;")
+    o("    const int tpw = 64 / (%d * a->team_groups);" % G)
+    o("    const size_t reg = (size_t)((tpw * %d > 64 * %d) ? tpw * %d : 64 * %d);" % (NE * NCD, NS, NE * NCD, NS))
+    o("    lds = (size_t)(block / 64) * reg * sizeof(double);")
     o("    if (a->boys_mode == 0)")
     o("        hipLaunchKernelGGL(%s, dim3(grid), dim3(block), lds, st, *a);" % name)
     o("    else")
