@@ -71,6 +71,15 @@ def cpu_baseline(batch, n_sample, seed=7, target_s=12.0, chunk=2_000_000):
             "quartets_per_s": float(n_q / dt)}
 
 
+def kernel_label(b):
+    """Name of the kernel a spherical FP64 bin launches with the built-in
+    matrices: the generated class kernel's `_bs` form, or the workgroup
+    kernel (bins with 64-lane groups)."""
+    if b.get("G") == 64:
+        return "intc_eri_wg_k (workgroup kernel, class %d%d%d%d)" % tuple(b["cls"])
+    return "intc_eri_d%d%d%d%d_s_bs" % tuple(b["cls"])
+
+
 def measured_traffic(kernel, workload):
     """HBM bytes per launch of `kernel` from the committed counter runs
     (profiles/traffic.json: FETCH_SIZE + WRITE_SIZE, scaled by a 1 GiB copy),
@@ -208,7 +217,7 @@ def run_molecule(args, three_center=False):
     per_bin = _per_bin(torch, plan, out64, flops_q, canon, stream)
     dom = max(per_bin, key=lambda r: r["ms"])
     achieved = dom["flops"] / (dom["ms"] * 1e-3) / 1e12
-    dom_name = "intc_eri_d%d%d%d%d_s" % tuple(dom["cls"])
+    dom_name = kernel_label(dom)
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(batch, args.cpu_sample)
@@ -349,7 +358,7 @@ def run_config1(args):
         per_bin.append(dict(b, ms=ms, flops=float(flops_q[sel].sum()), integrals=int(batch.sizes()[sel].sum())))
     dom = max(per_bin, key=lambda r: r["ms"])
     achieved = dom["flops"] / (dom["ms"] * 1e-3) / 1e12
-    dom_name = "intc_eri_d%d%d%d%d_s" % tuple(dom["cls"])
+    dom_name = kernel_label(dom)
     total_flops = float(flops_q.sum())
 
     # parity spot check against the oracle (Chebyshev Boys vs accurate Boys)
@@ -444,7 +453,7 @@ def traffic_probe(args):
     torch.cuda.synchronize()
     nq = sum(r["nq"] for r in hit)
     plan.close()
-    return {"probe": "class kernel intc_eri_d%d%d%d%d_s once (%d quartets) + torch.add over 2^27 doubles "
+    return {"probe": "class kernel intc_eri_d%d%d%d%d_s_bs once (%d quartets) + torch.add over 2^27 doubles "
                      "(1 GiB read, 1 GiB written)" % (cls + (nq,)), "quartets": nq}
 
 

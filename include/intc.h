@@ -101,7 +101,8 @@ int intc_plan_bin_info(const intc_plan *plan, int bin, int prec, int *cls4, int 
                        int *G);
 int intc_plan_set_bin_tuning(intc_plan *plan, int bin, int prec, int team_groups, int grid);
 /* Device pointers.  sph: S_l for l = 0..7 concatenated, each ncc(l) x (2l+1)
- * column-major (NULL for Cartesian plans).  stream: hipStream_t or NULL. */
+ * column-major; NULL selects the built-in real solid harmonics (compiled into
+ * the kernels; ignored by Cartesian plans).  stream: hipStream_t or NULL. */
 int intc_plan_build_pairs(intc_plan *plan, const double *exps, const double *coefs, const double *shell_cen,
                           void *stream);
 /* One class bin; the pair data must have been built (build_pairs or execute). */

@@ -42,13 +42,13 @@ def _digest(paths, extra=""):
     return h.hexdigest()
 
 
-def _compile(src, deps):
+def _compile(src, deps, force=False):
     os.makedirs(OBJ, exist_ok=True)
     base = os.path.splitext(os.path.basename(src))[0]
     obj = os.path.join(OBJ, base + ".o")
     stamp = obj + ".sha"
     dig = _digest([src] + deps, " ".join(FLAGS))
-    if os.path.exists(obj) and os.path.exists(stamp) and open(stamp).read() == dig:
+    if not force and os.path.exists(obj) and os.path.exists(stamp) and open(stamp).read() == dig:
         return obj, False
     cmd = [HIPCC] + FLAGS + ["-I", CSRC, "-c", src, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -59,7 +59,10 @@ def _compile(src, deps):
     return obj, True
 
 
-def build(lmax=2, jobs=None, verbose=True):
+def build(lmax=2, jobs=None, verbose=True, force=False):
+    """Generate the class kernels, compile every object (all of them when
+    ``force``, else only those whose sources or flags changed) and link
+    libintc.so."""
     from . import boys_tables, codegen
     inc = os.path.join(CSRC, "boys_tables.inc")
     if not os.path.exists(inc):
@@ -75,7 +78,7 @@ def build(lmax=2, jobs=None, verbose=True):
     srcs.append((os.path.join(CSRC, "intc_eri_wg.hip"), common))
     jobs = jobs or min(16, os.cpu_count() or 4)
     with ThreadPoolExecutor(jobs) as ex:
-        res = list(ex.map(lambda a: _compile(*a), srcs))
+        res = list(ex.map(lambda a: _compile(*a, force=force), srcs))
     objs = [r[0] for r in res]
     rebuilt = sum(1 for r in res if r[1])
     cmd = [HIPCC, "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", LIB + ".tmp"] + objs

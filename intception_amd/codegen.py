@@ -33,7 +33,12 @@ import os
 
 import numpy as np
 
-from .basis import ncc, ncc0, nsph, cart_components
+from .basis import SPH, ncc, ncc0, nsph, cart_components, sph_matrix
+
+# built-in S_l (real solid harmonics), l = 0..7: compiled into the "_bs"
+# kernels as literals and into the runtime as the device table used when a
+# caller passes no matrices
+SPH_BUILTIN = {l: (SPH[l] if l in SPH else sph_matrix(l)) for l in range(8)}
 
 
 def comps_upto(l):
@@ -250,7 +255,10 @@ def emit_class_source(cls, spherical, prec="d", waves_per_eu=None):
         o("    {%s}," % ", ".join(str(v) for v in r))
     o("};")
     o("")
-    o("template <int BM>")
+    # BM: Boys scheme (0 Chebyshev, 1 reference); BS: 1 = the built-in real
+    # solid-harmonic matrices are compiled in as literals (zero terms dropped),
+    # 0 = the caller's matrices are read from a.sph
+    o("template <int BM, int BS>")
     o("__device__ __forceinline__ void %s_body(const IntcEriArgs &a)" % name)
     o("{")
     o.ind = 1
@@ -446,6 +454,20 @@ def emit_class_source(cls, spherical, prec="d", waves_per_eu=None):
         o(ln)
     cc, dc = cart_components(LC), cart_components(LD)
     if spherical:
+        o("if constexpr (BS) {")
+        o.ind += 1
+        # d first: K1(c, sd) = sum_d H(c,d) S_LD[d, sd], nonzero terms only
+        for ci, c in enumerate(cc):
+            for sd in range(nsph(LD)):
+                terms = _sparse_terms([H(c, d) for d in dc], SPH_BUILTIN[LD][:, sd])
+                o("const double k1_%d_%d = %s;" % (ci, sd, terms))
+        for sc in range(nsph(LC)):
+            for sd in range(nsph(LD)):
+                terms = _sparse_terms(["k1_%d_%d" % (ci, sd) for ci in range(len(cc))], SPH_BUILTIN[LC][:, sc])
+                o("tlds[(e - %d) * NCD + %d] = %s;" % (eoff, sc * nD + sd, terms))
+        o.ind -= 1
+        o("} else {")
+        o.ind += 1
         sC, sD = "a.sph + %d" % intc_sph_off(LC), "a.sph + %d" % intc_sph_off(LD)
         o("const double *SC = %s, *SD = %s;" % (sC, sD))
         # d first: K1(c, sd) = sum_d H(c,d) SD[d + sd*ncc(LD)]
@@ -457,6 +479,8 @@ def emit_class_source(cls, spherical, prec="d", waves_per_eu=None):
             for sd in range(nsph(LD)):
                 terms = " + ".join("SC[%d] * k1_%d_%d" % (ci + sc * ncc(LC), ci, sd) for ci in range(len(cc)))
                 o("tlds[(e - %d) * NCD + %d] = %s;" % (eoff, sc * nD + sd, terms))
+        o.ind -= 1
+        o("}")
     else:
         for ci, c in enumerate(cc):
             for di, d in enumerate(dc):
@@ -481,6 +505,19 @@ def emit_class_source(cls, spherical, prec="d", waves_per_eu=None):
         o(ln)
     ac, bc = cart_components(LA), cart_components(LB)
     if spherical:
+        o("if constexpr (BS) {")
+        o.ind += 1
+        for ai, aa in enumerate(ac):
+            for sb in range(nsph(LB)):
+                terms = _sparse_terms([HB(aa, bb) for bb in bc], SPH_BUILTIN[LB][:, sb])
+                o("const double m1_%d_%d = %s;" % (ai, sb, terms))
+        for sa in range(nsph(LA)):
+            for sb in range(nsph(LB)):
+                terms = _sparse_terms(["m1_%d_%d" % (ai, sb) for ai in range(len(ac))], SPH_BUILTIN[LA][:, sa])
+                o("op[%d * sA + %d * sB] = %s;" % (sa, sb, terms))
+        o.ind -= 1
+        o("} else {")
+        o.ind += 1
         o("const double *SA = a.sph + %d, *SB = a.sph + %d;" % (intc_sph_off(LA), intc_sph_off(LB)))
         for ai, aa in enumerate(ac):
             for sb in range(nsph(LB)):
@@ -490,6 +527,8 @@ def emit_class_source(cls, spherical, prec="d", waves_per_eu=None):
             for sb in range(nsph(LB)):
                 terms = " + ".join("SA[%d] * m1_%d_%d" % (ai + sa * ncc(LA), ai, sb) for ai in range(len(ac)))
                 o("op[%d * sA + %d * sB] = %s;" % (sa, sb, terms))
+        o.ind -= 1
+        o("}")
     else:
         for ai, aa in enumerate(ac):
             for bi, bb in enumerate(bc):
@@ -506,10 +545,13 @@ def emit_class_source(cls, spherical, prec="d", waves_per_eu=None):
     # waves_per_eu: occupancy target for the Chebyshev-mode kernel (the compiler
     # then caps its registers at 512 / waves VGPRs)
     occ = " __attribute__((amdgpu_waves_per_eu(%d)))" % waves_per_eu if waves_per_eu else ""
-    o('extern "C" __global__ void __launch_bounds__(256)%s %s(const IntcEriArgs a) { %s_body<0>(a); }'
+    o('extern "C" __global__ void __launch_bounds__(256)%s %s(const IntcEriArgs a) { %s_body<0, 0>(a); }'
       % (occ, name, name))
-    o('extern "C" __global__ void __launch_bounds__(256) %s_refboys(const IntcEriArgs a) { %s_body<1>(a); }'
+    o('extern "C" __global__ void __launch_bounds__(256) %s_refboys(const IntcEriArgs a) { %s_body<1, 0>(a); }'
       % (name, name))
+    if spherical:
+        o('extern "C" __global__ void __launch_bounds__(256)%s %s_bs(const IntcEriArgs a) { %s_body<0, 1>(a); }'
+          % (occ, name, name))
     o('extern "C" hipError_t %s_launch(const IntcEriArgs *a, int grid, int block, size_t lds, hipStream_t st)'
       % name)
     o("{")
@@ -518,7 +560,12 @@ def emit_class_source(cls, spherical, prec="d", waves_per_eu=None):
     o("    const int tpw = 64 / (%d * a->team_groups);" % G)
     o("    const size_t reg = (size_t)((tpw * %d > 64 * %d) ? tpw * %d : 64 * %d);" % (NE * NCD, NS, NE * NCD, NS))
     o("    lds = (size_t)(block / 64) * reg * sizeof(double);")
-    o("    if (a->boys_mode == 0)")
+    if spherical:
+        o("    if (a->boys_mode == 0 && a->sph_builtin)")
+        o("        hipLaunchKernelGGL(%s_bs, dim3(grid), dim3(block), lds, st, *a);" % name)
+        o("    else if (a->boys_mode == 0)")
+    else:
+        o("    if (a->boys_mode == 0)")
     o("        hipLaunchKernelGGL(%s, dim3(grid), dim3(block), lds, st, *a);" % name)
     o("    else")
     o("        hipLaunchKernelGGL(%s_refboys, dim3(grid), dim3(block), lds, st, *a);" % name)
@@ -526,6 +573,23 @@ def emit_class_source(cls, spherical, prec="d", waves_per_eu=None):
     o("}")
     meta = dict(cls=cls, spherical=spherical, G=G, NE=NE, NCD=NCD, M=M, name=name, prec=prec)
     return o.text() + "\n", meta
+
+
+def _sparse_terms(xs, col):
+    """sum_i xs[i] * col[i] over the nonzero col[i], coefficients as exact
+    double literals (1.0 and -1.0 become plain terms)."""
+    out = []
+    for x, v in zip(xs, col):
+        v = float(v)
+        if v == 0.0:
+            continue
+        if v == 1.0:
+            out.append(x)
+        elif v == -1.0:
+            out.append("-" + x)
+        else:
+            out.append("%s * %s" % (x, repr(v)))
+    return " + ".join(out) if out else "0.0"
 
 
 def intc_sph_off(l):
@@ -604,6 +668,12 @@ def generate(outdir, classes, sph_modes=(False, True), precs=("d", "f"), wg=(), 
             n += 1
     lines.append("};")
     lines.append("static const int intc_n_class = %d;" % n)
+    # built-in S_l, l = 0..7, column-major, concatenated (the a.sph layout)
+    blob = np.concatenate([SPH_BUILTIN[l].reshape(-1, order="F") for l in range(8)])
+    lines.append("static const double intc_builtin_sph[%d] = {" % len(blob))
+    for i in range(0, len(blob), 4):
+        lines.append("    " + ", ".join(repr(float(v)) for v in blob[i:i + 4]) + ",")
+    lines.append("};")
     tab = os.path.join(outdir, "class_table.inc")
     src = "\n".join(lines) + "\n"
     old = open(tab).read() if os.path.exists(tab) else None

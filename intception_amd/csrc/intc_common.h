@@ -43,6 +43,7 @@ struct IntcEriArgs {
     const double *boys_medium;        // [201][7]
     double *out;
     int wg_cls[6];                    // workgroup kernels: la, lb, lc, ld, spherical (canonical class)
+    int sph_builtin;                  // 1: sph is the built-in table (register kernels use literals)
 };
 
 // offset of S_l inside IntcEriArgs::sph: sum_{l' < l} ncc(l') (2l' + 1)

@@ -98,16 +98,27 @@ __global__ void __launch_bounds__(256) intc_mlp_features_k(long long n, const fl
 
 // ---------------------------------------------------------------------------
 // inference + routing on MFMA.  One wave per super-tile of 64 quartets: lane
-// l computes (once) the standardised features of quartet 64 t + l into LDS,
-// then the wave runs the network on the four 16-quartet tiles.
-// v_mfma_f32_16x16x16_bf16 operands (lane l, slot j = 0..3):
-//   A[i][k]: i = l % 16, k = 4 (l / 16) + j
-//   B[k][n]: n = l % 16, k = 4 (l / 16) + j
-//   D[i][n]: n = l % 16, i = 4 (l / 16) + j
+// l computes the standardised features of quartet 64 t + l into LDS, then the
+// wave runs the network on the four 16-quartet tiles, transposed (hidden
+// units as rows, quartets as columns) so that each layer's accumulator is the
+// next layer's B operand in registers:
+//   H1^T = relu(W1 X^T + b1)      v_mfma_f32_16x16x16_bf16, 4 row tiles
+//   H2^T = relu(W2 H1^T + b2)     v_mfma_f32_16x16x32_bf16, 4 row tiles x 2 K chunks
+//   y    = w3 . bf16(H2^T) + b3   per lane over its 16 hidden units, then
+//                                 across the 4 lane groups
+// Operands (lane l, r = l % 16, g = l / 16, slot s):
+//   A[i][k]: i = r; B[k][n]: n = r; D[i][n]: n = r, i = 4 g + s (s < 4).
+// The K index of a slot is the same function of (g, s) in A and B, so the
+// K chunks of layer 2 may order the hidden-1 units freely: chunk c, slot s
+// <-> unit 16 (2c + s / 4) + 4 g + s % 4, the unit layer 1's D holds there.
 // route: 2 if pred + m_skip < log_skip, else 1 if pred + m_32 < log_32 and
 // log10 Q_bra + log10 Q_ket < log_qq32, else 0.
 // ---------------------------------------------------------------------------
-constexpr int SF = KF + 1;  // LDS row: 16 standardised features (zero padded) + log10 Q_bra Q_ket
+constexpr int SF = KF + 1;  // LDS row: 16 standardised features (zero padded), padded to 17 words
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ float bf16_round(float x) { return (float)(__bf16)x; }
 
 __global__ void __launch_bounds__(256) intc_mlp_route_k(long long n, const float *__restrict__ pf,
                                                         const int *__restrict__ qp, const float *__restrict__ norm,
@@ -116,106 +127,83 @@ __global__ void __launch_bounds__(256) intc_mlp_route_k(long long n, const float
                                                         float log_skip, float m_32, float log_32, float log_qq32,
                                                         float *__restrict__ pred, signed char *__restrict__ route)
 {
-    __shared__ uint16_t sW[NB16];
-    __shared__ float sB[2 * NH + NH + 1];
-    __shared__ float sH[4][16][NH + 1];
     __shared__ float sF[4][64][SF];
-    for (int i = threadIdx.x; i < NB16; i += blockDim.x) sW[i] = wb[i];
-    for (int i = threadIdx.x; i < NH; i += blockDim.x) {
-        sB[i] = master[OB1 + i];
-        sB[NH + i] = master[OB2 + i];
-        sB[2 * NH + i] = master[OW3 + i];
-    }
-    if (threadIdx.x == 0) sB[3 * NH] = master[OB3];
-    __syncthreads();
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int r16 = lane & 15, g4 = lane >> 4;
-    // B operands of layers 1 and 2 stay in registers for every tile
-    bf16x4 b1w[4], b2w[4][4];
+    // A operands of layers 1 and 2 (rows 16 nt + r16), and the biases and
+    // W3 entries of the hidden units this lane's accumulator slots hold
+    bf16x4 a1w[4];
+    bf16x8 a2w[4][2];
+    float b1r[4][4], b2r[4][4], w3r[4][4];
     for (int nt = 0; nt < 4; ++nt) {
-        for (int j = 0; j < 4; ++j) b1w[nt][j] = (short)sW[BW1 + (nt * 16 + r16) * KF + 4 * g4 + j];
-        for (int ks = 0; ks < 4; ++ks)
-            for (int j = 0; j < 4; ++j)
-                b2w[ks][nt][j] = (short)sW[BW2 + (nt * 16 + r16) * NH + ks * 16 + 4 * g4 + j];
+        const int row = 16 * nt + r16;
+        for (int j = 0; j < 4; ++j) a1w[nt][j] = (short)wb[BW1 + row * KF + 4 * g4 + j];
+        for (int c = 0; c < 2; ++c)
+            for (int sl = 0; sl < 8; ++sl)
+                a2w[nt][c][sl] = __builtin_bit_cast(__bf16, wb[BW2 + row * NH + 16 * (2 * c + sl / 4) + 4 * g4 + sl % 4]);
+        for (int j = 0; j < 4; ++j) {
+            const int h = 16 * nt + 4 * g4 + j;
+            b1r[nt][j] = master[OB1 + h];
+            b2r[nt][j] = master[OB2 + h];
+            w3r[nt][j] = bf2f(wb[BW3 + h]);
+        }
     }
+    const float b3 = master[OB3];
     const long long nsuper = (n + 63) / 64;
     const long long stride = (long long)gridDim.x * (blockDim.x >> 6);
     for (long long t = (long long)blockIdx.x * (blockDim.x >> 6) + wave; t < nsuper; t += stride) {
-        {
-            const long long q = t * 64 + lane;
-            const long long qc = q < n ? q : n - 1;
-            float f[NF];
-            quartet_features(pf, qp[2 * qc], qp[2 * qc + 1], f);
-            for (int k = 0; k < NF; ++k) sF[wave][lane][k] = (f[k] - norm[k]) * norm[NF + k];
-            for (int k = NF; k < KF; ++k) sF[wave][lane][k] = 0.0f;
-            sF[wave][lane][KF] = f[0] + f[1];
+        const long long q = t * 64 + lane;
+        const long long qc = q < n ? q : n - 1;
+        float f[NF];
+        quartet_features(pf, qp[2 * qc], qp[2 * qc + 1], f);
+        for (int k = 0; k < NF; ++k) sF[wave][lane][k] = (f[k] - norm[k]) * norm[NF + k];
+        for (int k = NF; k < KF; ++k) sF[wave][lane][k] = 0.0f;
+        const float lq = f[0] + f[1];
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        float mine = 0.0f;  // network output of quartet 64 t + lane
+        for (int tt = 0; tt < 4; ++tt) {
+            if (t * 64 + 16 * tt >= n) break;  // uniform over the wave
+            // B of layer 1: X^T, features 4 g4 + j of quartet 16 tt + r16
+            bf16x4 bx;
+            for (int j = 0; j < 4; ++j) bx[j] = (short)f2bf(sF[wave][16 * tt + r16][4 * g4 + j]);
+            f32x4 d1[4];
+            for (int nt = 0; nt < 4; ++nt)
+                d1[nt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a1w[nt], bx, f32x4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+            // B of layer 2: relu(H1) in BF16, K chunk c slot s = unit 16 (2c + s/4) + 4 g4 + s%4
+            bf16x8 h1[2];
+            for (int c = 0; c < 2; ++c)
+                for (int sl = 0; sl < 8; ++sl) {
+                    const float h = d1[2 * c + sl / 4][sl % 4] + b1r[2 * c + sl / 4][sl % 4];
+                    h1[c][sl] = (__bf16)(h > 0.f ? h : 0.f);
+                }
+            float part = 0.0f;
+            for (int nt = 0; nt < 4; ++nt) {
+                f32x4 d2 = {0.f, 0.f, 0.f, 0.f};
+                d2 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2w[nt][0], h1[0], d2, 0, 0, 0);
+                d2 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2w[nt][1], h1[1], d2, 0, 0, 0);
+                for (int j = 0; j < 4; ++j) {
+                    const float h = d2[j] + b2r[nt][j];
+                    part += bf16_round(h > 0.f ? h : 0.f) * w3r[nt][j];
+                }
+            }
+            part += __shfl_xor(part, 16);
+            part += __shfl_xor(part, 32);
+            if (g4 == tt) mine = part;  // lane 16 tt + r16 keeps quartet 16 tt + r16
+        }
+        if (q < n) {
+            // the network predicts log10(max|integral| / (Q_bra Q_ket)) <= 0
+            const float y = lq + fminf(mine + b3, 0.0f);
+            if (pred) pred[q] = y;
+            if (route) {
+                signed char r = 0;
+                if (y + m_skip < log_skip) r = 2;
+                else if (y + m_32 < log_32 && lq < log_qq32) r = 1;
+                route[q] = r;
+            }
         }
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         __builtin_amdgcn_wave_barrier();
-        for (int tt = 0; tt < 4; ++tt) {
-            const long long row0 = t * 64 + 16 * tt;
-            if (row0 >= n) break;  // uniform over the wave
-            // layer-1 A operand: row r16 of this tile, features 4*g4 .. 4*g4+3
-            bf16x4 a1;
-            for (int j = 0; j < 4; ++j) a1[j] = (short)f2bf(sF[wave][16 * tt + r16][4 * g4 + j]);
-            // layer 1
-            for (int nt = 0; nt < 4; ++nt) {
-                f32x4 d = {0.f, 0.f, 0.f, 0.f};
-                d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a1, b1w[nt], d, 0, 0, 0);
-                for (int j = 0; j < 4; ++j) {
-                    float h = d[j] + sB[nt * 16 + r16];
-                    sH[wave][4 * g4 + j][nt * 16 + r16] = h > 0.f ? h : 0.f;
-                }
-            }
-            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            // layer 2: K = 64 in 4 steps
-            f32x4 acc[4];
-            for (int nt = 0; nt < 4; ++nt) acc[nt] = f32x4{0.f, 0.f, 0.f, 0.f};
-            for (int ks = 0; ks < 4; ++ks) {
-                bf16x4 a;
-                for (int j = 0; j < 4; ++j) a[j] = (short)f2bf(sH[wave][r16][ks * 16 + 4 * g4 + j]);
-                for (int nt = 0; nt < 4; ++nt)
-                    acc[nt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b2w[ks][nt], acc[nt], 0, 0, 0);
-            }
-            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            // layer 3: rows 4*g4 + j, columns r16 + 16 nt, reduced over r16
-            float part[4];
-            for (int j = 0; j < 4; ++j) {
-                float sacc = 0.f;
-                for (int nt = 0; nt < 4; ++nt) {
-                    int col = nt * 16 + r16;
-                    float h = acc[nt][j] + sB[NH + col];
-                    h = h > 0.f ? h : 0.f;
-                    sacc += bf2f(f2bf(h)) * bf2f(sW[BW3 + col]);
-                }
-                part[j] = sacc;
-            }
-            for (int off = 1; off < 16; off <<= 1)
-                for (int j = 0; j < 4; ++j) part[j] += __shfl_xor(part[j], off);
-            // the lane with r16 == j writes row 4*g4 + j
-            for (int j = 0; j < 4; ++j) {
-                const int row = 4 * g4 + j;
-                if (r16 == j) {
-                    const long long qq = row0 + row;
-                    if (qq < n) {
-                        const float lq = sF[wave][16 * tt + row][KF];
-                        // the network predicts log10(max|integral| / (Q_bra Q_ket)) <= 0
-                        const float y = lq + fminf(part[j] + sB[3 * NH], 0.0f);
-                        if (pred) pred[qq] = y;
-                        if (route) {
-                            signed char r = 0;
-                            if (y + m_skip < log_skip) r = 2;
-                            else if (y + m_32 < log_32 && lq < log_qq32) r = 1;
-                            route[qq] = r;
-                        }
-                    }
-                }
-            }
-            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-        }
     }
 }
 

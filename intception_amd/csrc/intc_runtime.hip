@@ -43,6 +43,7 @@ int fail(int code, const std::string &msg)
 
 struct DevTables {
     double *cheb = nullptr, *small = nullptr, *medium = nullptr;
+    double *sph = nullptr;  // built-in S_l, l = 0..7
 };
 std::mutex g_mu;
 std::vector<DevTables> g_tables;  // per device
@@ -66,6 +67,8 @@ int ensure_tables(int dev, DevTables **out)
         HIPCHK(hipMemcpy(t.cheb, intc_boys_cheb_host, sizeof(intc_boys_cheb_host), hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(t.small, intc_boys_smallx_host, sizeof(intc_boys_smallx_host), hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(t.medium, intc_boys_mediumx_host, sizeof(intc_boys_mediumx_host), hipMemcpyHostToDevice));
+        HIPCHK(hipMalloc(&t.sph, sizeof(intc_builtin_sph)));
+        HIPCHK(hipMemcpy(t.sph, intc_builtin_sph, sizeof(intc_builtin_sph), hipMemcpyHostToDevice));
         if (intc_wg_prepare()) return fail(-3, "workgroup kernels: cannot enable their LDS size");
     }
     *out = &t;
@@ -848,7 +851,10 @@ static int fill_args(const intc_plan *p, const double *sph, double *out, IntcEri
     a->pp_py = p->ppy;
     a->pp_pz = p->ppz;
     a->pp_kappa = p->pkap;
-    a->sph = sph;
+    // no caller matrices: the built-in real solid harmonics (the register
+    // kernels then use their compiled-in copy)
+    a->sph = sph ? sph : t->sph;
+    a->sph_builtin = sph ? 0 : 1;
     a->boys_cheb = t->cheb;
     a->boys_small = t->small;
     a->boys_medium = t->medium;
@@ -874,7 +880,6 @@ INTC_API int intc_plan_run_bin(intc_plan *p, int i, int prec, const double *sph,
 {
     if (i < 0 || i >= (int)p->bins.size()) return fail(-1, "bin index out of range");
     if (prec != 0 && prec != 1) return fail(-1, "prec must be 0 (FP64) or 1 (FP32 recurrences)");
-    if (p->spherical && !sph) return fail(-1, "spherical plan needs the S matrices");
     if (!p->pairs_built && p->n_pp > 0)
         return fail(-1, "pair data not built: call intc_plan_build_pairs (or execute) first");
     IntcEriArgs a;
@@ -973,7 +978,6 @@ static int ensure_streams(intc_plan *p, int n_streams)
 INTC_API int intc_plan_execute(intc_plan *p, const double *exps, const double *coefs, const double *shell_cen,
                                const double *sph, double *out, void *stream)
 {
-    if (p->spherical && !sph) return fail(-1, "spherical plan needs the S matrices");
     int rc = intc_plan_build_pairs(p, exps, coefs, shell_cen, stream);
     if (rc) return rc;
     return run_all(p, sph, out, nullptr, 1, (hipStream_t)stream);
@@ -983,7 +987,6 @@ INTC_API int intc_plan_execute_routed(intc_plan *p, const double *exps, const do
                                       const double *shell_cen, const double *sph, const signed char *route,
                                       double *out, void *stream)
 {
-    if (p->spherical && !sph) return fail(-1, "spherical plan needs the S matrices");
     if (!route) return fail(-1, "route array is required");
     int rc = intc_plan_build_pairs(p, exps, coefs, shell_cen, stream);
     if (rc) return rc;
@@ -999,7 +1002,6 @@ INTC_API int intc_plan_graph_create(intc_plan *p, const double *exps, const doub
                                     void *stream)
 {
     if (n_streams < 1 || n_streams > 16) return fail(-1, "n_streams must be in [1, 16]");
-    if (p->spherical && !sph) return fail(-1, "spherical plan needs the S matrices");
     if (p->graph_exec) {
         (void)hipGraphExecDestroy(p->graph_exec);
         p->graph_exec = nullptr;

@@ -35,7 +35,10 @@ def _ptr(t):
 
 
 class EriPlan:
-    def __init__(self, batch, device="cuda"):
+    def __init__(self, batch, device="cuda", sph=None):
+        """sph: None runs spherical plans with the built-in real solid
+        harmonics (compiled into the kernels); a blob in the sph_blob()
+        layout is uploaded and read by the kernels instead."""
         torch = _torch()
         L = _lib.load()
         self.batch = batch
@@ -68,7 +71,9 @@ class EriPlan:
         self.d_exps = torch.from_numpy(np.ascontiguousarray(b.exps, dtype=np.float64)).to(self.device)
         self.d_coefs = torch.from_numpy(np.ascontiguousarray(b.coefs, dtype=np.float64)).to(self.device)
         self.d_cen = torch.from_numpy(cen.reshape(-1)).to(self.device)
-        self.d_sph = torch.from_numpy(sph_blob()).to(self.device) if b.spherical else None
+        self.d_sph = None
+        if b.spherical and sph is not None:
+            self.d_sph = torch.from_numpy(np.ascontiguousarray(sph, dtype=np.float64)).to(self.device)
         self.out_size = int(b.out_size)
 
     def bins(self, prec=0):
@@ -167,10 +172,11 @@ def set_boys_mode(mode):
     _lib.check(_lib.load().intc_set_boys_mode(int(mode)))
 
 
-def compute(batch, device="cuda"):
-    """Convenience: all integrals of a batch as a host numpy array."""
+def compute(batch, device="cuda", sph=None):
+    """Convenience: all integrals of a batch as a host numpy array (sph as
+    for EriPlan)."""
     torch = _torch()
-    plan = EriPlan(batch, device)
+    plan = EriPlan(batch, device, sph=sph)
     out = plan.run()
     torch.cuda.synchronize(plan.device)
     res = out.cpu().numpy()

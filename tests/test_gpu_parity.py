@@ -44,6 +44,28 @@ def test_config1_sample_spherical(gpu, mode):
     assert_parity(_gpu(b, mode), _oracle(b, mode), what="config1 mode %d" % mode)
 
 
+def test_caller_matrices_match_builtin(gpu):
+    # spherical output through the caller-matrix kernels (S read from device
+    # memory, dense sums) and through the built-in ones (S compiled in as
+    # literals, zero terms dropped): both to the oracle bar, and to each
+    # other within rounding of the dropped-term reordering
+    import itertools
+    cls = np.array(list(itertools.product(range(3), repeat=4)), dtype=np.int32)
+    b = basis.make_random_quartets(515, np.repeat(cls, 2, axis=0), [1, 2, 3], True)
+    print("caller vs built-in S: %d quartets" % b.n_quartet, flush=True)
+    ref = _oracle(b, 0)
+    caller = runtime.compute(b, sph=runtime.sph_blob())
+    builtin = runtime.compute(b)
+    assert_parity(caller, ref, what="caller S")
+    assert_parity(builtin, ref, what="built-in S")
+    # per quartet: |built-in - caller| <= 1e-13 x block max (measured: up to
+    # 2.5e-14; sums associated differently cancel inside a block, while a
+    # wrong or dropped coefficient moves a block by O(block max))
+    bm = np.maximum.reduceat(np.abs(caller), b.out_off)
+    dm = np.maximum.reduceat(np.abs(builtin - caller), b.out_off)
+    assert np.all(dm <= 1e-13 * bm), np.max(dm / np.maximum(bm, 1e-300))
+
+
 def test_all_81_classes_each_permutation(gpu):
     # every (la lb|lc ld) with l <= 2, Cartesian and spherical, K in 1..3
     import itertools
