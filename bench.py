@@ -126,11 +126,14 @@ def run_molecule(args, three_center=False):
     t0 = time.perf_counter()
     n_atoms = args.atoms or (100 if three_center else 200)
     mol_seed = 202604 if three_center else 202603
-    train_seeds, val_seed = (((202621, 202622), (202629, 202628)) if three_center else
-                             ((202611, 202612), (202619, 202618)))
+    # three-centre quartets: a lower skip threshold and three held-out
+    # molecules -- the margin calibrated at 5e-11 on two molecules did not hold
+    # over the full (ab|P s) set (max error 1.4e-10)
+    train_seeds, val_seed, tau_skip = (((202621, 202622), (202629, 202628, 202627), 2.5e-11) if three_center else
+                                       ((202611, 202612), (202619, 202618), 5e-11))
     router, rstats = screening.build_router(n_train=args.train, n_atoms=n_atoms, epochs=15, n_val=1_000_000,
                                             log=log, three_center=three_center, train_seeds=train_seeds,
-                                            val_seed=val_seed)
+                                            val_seed=val_seed, tau_skip=tau_skip)
     t_router = time.perf_counter() - t0
     t0 = time.perf_counter()
     mol, pf = screening.molecule_setup(n_atoms, mol_seed, three_center=three_center)
@@ -245,9 +248,9 @@ def run_molecule(args, three_center=False):
                                  "passing Schwarz 1e-12; spherical")) % (n_atoms, mol.radius, batch.n_quartet),
                    "quartets": int(batch.n_quartet), "integrals": int(n_int), "seed": mol_seed,
                    "router_training": "%d quartets from molecules seeded %d/%d, calibrated on %d quartets of "
-                                      "molecules seeded %d/%d" % (rstats["train_samples"], train_seeds[0],
-                                                                  train_seeds[1], rstats["val_samples"],
-                                                                  val_seed[0], val_seed[1])},
+                                      "molecules seeded %s" % (rstats["train_samples"], train_seeds[0],
+                                                               train_seeds[1], rstats["val_samples"],
+                                                               "/".join(str(v) for v in val_seed))},
         "plain_fp64": {"ms_per_step": float(ms64), "integrals_per_s": float(n_int / (ms64 * 1e-3)),
                        "quartets_per_s": float(batch.n_quartet / (ms64 * 1e-3))},
         "learned_skip_fp64": learned_skip,

@@ -24,9 +24,16 @@ FLAGS = ["-O3", "-fPIC", "-std=c++17", "--offload-arch=" + ARCH,
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
          "-Wno-unused-command-line-argument"]
 
-# occupancy targets (waves per SIMD) for class kernels that reach them without
-# spilling: the compiler otherwise settles at one wave fewer
-OCCUPANCY = {}
+# occupancy targets (waves per SIMD) for class kernels that reach them with at
+# most a few bytes of spill and run faster there (config-3 bins on MI355X:
+# (ps|dp) 3.32 -> 3.06 ms, (ds|dp) 1.75 -> 1.64 ms); the compiler otherwise
+# settles at one wave fewer
+OCCUPANCY = {"intc_eri_d1021_s": 3, "intc_eri_d2021_s": 3}
+# batched-setup choice where it differs from the default rule (groups of >= 4
+# lanes, ket of 2-3 levels), measured per kernel on the config-3 bins: the
+# FP32-recurrence kernels of these classes gain from it
+# ((ps|ps) 2.81 -> 2.03 ms, (pp|dd) 2.03 -> 1.82 ms, (ds|dd), (dp|dd))
+BATCH = {"intc_eri_f1010_s": True, "intc_eri_f1122_s": True, "intc_eri_f2022_s": True, "intc_eri_f2122_s": True}
 
 # classes compiled into the library: every canonical class with shells up to
 # l = 2 (Cartesian and spherical), plus the three-centre set (ab|P s) with
@@ -69,7 +76,7 @@ def build(lmax=2, jobs=None, verbose=True, force=False):
         boys_tables.write_inc(inc)
     classes = codegen.class_set(lmax, EXTRA_CLASSES)
     wg = codegen.wg_classes(3, classes)
-    files, metas = codegen.generate(GEN, classes, wg=wg, occupancy=OCCUPANCY)
+    files, metas = codegen.generate(GEN, classes, wg=wg, occupancy=OCCUPANCY, batch=BATCH)
     common = [os.path.join(CSRC, "intc_common.h")]
     srcs = [(f, common) for f in files]
     srcs.append((os.path.join(CSRC, "intc_runtime.hip"),

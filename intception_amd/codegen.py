@@ -119,7 +119,8 @@ class Emitter:
 # groups of at least this many lanes whose ket reaches lc + ld in BATCH_LK use
 # the batched setup loop (measured on MI355X, config-3 bins: 1.06-1.37x for
 # ket 2-3; for a one-level ket the recurrences are too short to pay for the
-# LDS round trip, and the ket-4 kernels sit at the register limit)
+# LDS round trip, and the ket-4 kernels sit at the register limit); a
+# per-kernel choice (emit_class_source's batch) overrides the rule
 BATCH_MIN_G = 4
 BATCH_LK = (2, 3)
 
@@ -205,7 +206,7 @@ def _hrr_lines(src, lo, nmove, shift_vec, prefix, kind):
     return L, (lambda a_, b_: names[(a_, b_)])
 
 
-def emit_class_source(cls, spherical, prec="d", waves_per_eu=None):
+def emit_class_source(cls, spherical, prec="d", waves_per_eu=None, batch=None):
     """prec "d": FP64 throughout.  prec "f": the per-primitive recurrences (bra
     chain, ket VRR, cross-lane terms) in FP32 on the Boys values F_m; FP64 for
     the Boys function, the prefactor exp(-mu R^2) terms, the contraction sums
@@ -267,7 +268,7 @@ def emit_class_source(cls, spherical, prec="d", waves_per_eu=None):
     # primitive quartet kb + e into its LDS slot, then the group walks the G
     # slots.  The setup (pair loads, Boys, exp, divisions) is then issued once
     # per G primitive quartets instead of once per quartet in every lane.
-    batched = G >= BATCH_MIN_G and LR in BATCH_LK
+    batched = (G >= BATCH_MIN_G and LR in BATCH_LK) if batch is None else (bool(batch) and G >= 2)
     slot = []          # (name, C type in the recurrences) in slot order
     for m in range(M + 1):
         slot.append(("c0_%d" % m, R))
@@ -623,14 +624,14 @@ def _nout(l, sph):
     return 2 * l + 1 if sph else (l + 1) * (l + 2) // 2
 
 
-def generate(outdir, classes, sph_modes=(False, True), precs=("d", "f"), wg=(), occupancy=None):
+def generate(outdir, classes, sph_modes=(False, True), precs=("d", "f"), wg=(), occupancy=None, batch=None):
     os.makedirs(outdir, exist_ok=True)
     metas = []
     written = []
     for cls in classes:
         for sph, prec in itertools.product(sph_modes, precs):
             nm = kernel_name(cls, sph, prec)
-            src, meta = emit_class_source(cls, sph, prec, (occupancy or {}).get(nm))
+            src, meta = emit_class_source(cls, sph, prec, (occupancy or {}).get(nm), (batch or {}).get(nm))
             fn = os.path.join(outdir, meta["name"] + ".hip")
             old = open(fn).read() if os.path.exists(fn) else None
             if old != src:

@@ -52,10 +52,9 @@ __device__ __forceinline__ uint16_t f2bf(float f)
 
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
-// 12 unstandardised features of quartet (pb | pk) into f[0..11]
-__device__ __forceinline__ void quartet_features(const float *__restrict__ pf, int pb, int pk, float *f)
+// 12 unstandardised features of a quartet from its two pair-table rows a, c
+__device__ __forceinline__ void features_from_rows(const float *a, const float *c, float *f)
 {
-    const float *a = pf + 8 * (size_t)pb, *c = pf + 8 * (size_t)pk;
     float dx = a[4] - c[4], dy = a[5] - c[5], dz = a[6] - c[6];
     float r2 = dx * dx + dy * dy + dz * dz;
     float za = exp10f(a[1]), zc = exp10f(c[1]);
@@ -77,6 +76,31 @@ __device__ __forceinline__ void quartet_features(const float *__restrict__ pf, i
     const float f0 = T < 1e-6f ? 1.0f : 0.886226925f * erff(sqrtf(T)) / sqrtf(T);
     f[10] = log10f(f0) + 0.25f * log10f(4.0f * za * zc) - 0.5f * log10f(za + zc);
     f[11] = -(a[7] + c[7]) * log10f(1.0f + R * sqrtf(rho));
+}
+
+// 12 unstandardised features of quartet (pb | pk) into f[0..11]
+__device__ __forceinline__ void quartet_features(const float *__restrict__ pf, int pb, int pk, float *f)
+{
+    features_from_rows(pf + 8 * (size_t)pb, pf + 8 * (size_t)pk, f);
+}
+
+// the two pair-table rows of quartet q (clamped to n - 1), as float4s; the
+// table is 16-byte aligned (checked by the host)
+struct QuartetRows {
+    float4 a0, a1, c0, c1;
+};
+
+__device__ __forceinline__ int2 quartet_pairs_of(const int *__restrict__ qp, long long q, long long n)
+{
+    const long long qc = q < n ? q : n - 1;
+    return *reinterpret_cast<const int2 *>(qp + 2 * qc);
+}
+
+__device__ __forceinline__ QuartetRows rows_of(const float *__restrict__ pf, int2 pr)
+{
+    const float4 *a = reinterpret_cast<const float4 *>(pf + 8 * (size_t)pr.x);
+    const float4 *c = reinterpret_cast<const float4 *>(pf + 8 * (size_t)pr.y);
+    return QuartetRows{a[0], a[1], c[0], c[1]};
 }
 
 }  // namespace
@@ -151,11 +175,25 @@ __global__ void __launch_bounds__(256) intc_mlp_route_k(long long n, const float
     const float b3 = master[OB3];
     const long long nsuper = (n + 63) / 64;
     const long long stride = (long long)gridDim.x * (blockDim.x >> 6);
-    for (long long t = (long long)blockIdx.x * (blockDim.x >> 6) + wave; t < nsuper; t += stride) {
+    // software pipeline over this wave's super-tiles: the pair ids of tile
+    // t + 2 stride and the pair-table rows of tile t + stride are in flight
+    // while tile t is evaluated
+    long long t = (long long)blockIdx.x * (blockDim.x >> 6) + wave;
+    int2 pr_next = quartet_pairs_of(qp, (t + stride) * 64 + lane, n);
+    QuartetRows rows = rows_of(pf, quartet_pairs_of(qp, t * 64 + lane, n));
+    for (; t < nsuper; t += stride) {
         const long long q = t * 64 + lane;
-        const long long qc = q < n ? q : n - 1;
+        const QuartetRows cur = rows;
+        if (t + stride < nsuper) {
+            rows = rows_of(pf, pr_next);
+            pr_next = quartet_pairs_of(qp, (t + 2 * stride) * 64 + lane, n);
+        }
         float f[NF];
-        quartet_features(pf, qp[2 * qc], qp[2 * qc + 1], f);
+        {
+            const float a[8] = {cur.a0.x, cur.a0.y, cur.a0.z, cur.a0.w, cur.a1.x, cur.a1.y, cur.a1.z, cur.a1.w};
+            const float c[8] = {cur.c0.x, cur.c0.y, cur.c0.z, cur.c0.w, cur.c1.x, cur.c1.y, cur.c1.z, cur.c1.w};
+            features_from_rows(a, c, f);
+        }
         for (int k = 0; k < NF; ++k) sF[wave][lane][k] = (f[k] - norm[k]) * norm[NF + k];
         for (int k = NF; k < KF; ++k) sF[wave][lane][k] = 0.0f;
         const float lq = f[0] + f[1];
@@ -347,6 +385,8 @@ INTC_API int intc_mlp_route(long long n, const float *pair_feat, const int *quar
                             float log_32, float log_qq32, float *pred, signed char *route, void *stream)
 {
     if (n <= 0) return 0;
+    // the kernel reads pair-table rows as float4 and pair ids as int2
+    if (((uintptr_t)pair_feat & 15) || ((uintptr_t)quartet_pairs & 7)) return -1;
     long long supers = (n + 63) / 64;
     int grid = (int)std::min<long long>((supers + 3) / 4, 256 * 12);
     hipLaunchKernelGGL(intc_mlp_route_k, dim3(grid), dim3(256), 0, (hipStream_t)stream, n, pair_feat,

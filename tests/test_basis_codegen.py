@@ -96,6 +96,12 @@ def test_library_exports_every_declared_symbol():
     for n in names:
         assert hasattr(L, n), n
     assert {s[0] for s in lib.SIGNATURES} == names
+    # the network's entry points (include/intc_mlp.h)
+    hdr = re.sub(r"/\*.*?\*/", "", open(os.path.join(ROOT, "include", "intc_mlp.h")).read(), flags=re.S)
+    mlp_names = set(re.findall(r"\b(intc_mlp_\w+)\s*\(", hdr))
+    assert len(mlp_names) == 7
+    for n in mlp_names:
+        assert hasattr(L, n), n
     # host-only entry points work without a GPU
     assert L.intc_eri_cont_cart_work_array_size(1, 1, 1, 1, 3, 3, 3, 3, 1, 1, 1, 1) == 25930
     assert L.intc_set_boys_mode(7) != 0
