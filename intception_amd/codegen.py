@@ -123,23 +123,65 @@ class Emitter:
 # per-kernel choice (emit_class_source's batch) overrides the rule
 BATCH_MIN_G = 4
 BATCH_LK = (2, 3)
+# batched kernels: each lane reads its bra-chain step's PA / WP entry from the
+# setup slot (instead of selecting among the x, y, z entries)
+SLOT_PW = True
+
+
+def _lds_conflict(addrs):
+    """Largest number of distinct dwords that one LDS access maps to a single
+    bank (64 banks of 4 bytes)."""
+    banks = {}
+    for a in addrs:
+        banks.setdefault(a % 64, set()).add(a)
+    return max(len(v) for v in banks.values())
+
+
+def slot_stride(G, ns, ipa=None):
+    """Slot length in doubles for the batched setup: the smallest length
+    >= ns for which the wave's reads of a slot entry hit the fewest LDS bank
+    conflicts.  At step st the 64 // G groups of a wave read slot
+    (group * G + st): one entry each (broadcast within the group), or, for the
+    bra chain's PA / WP entries, one of the four at ipa.. per lane."""
+    ng = 64 // G
+    best = None
+    for n in range(ns, ns + 9):
+        worst = 0
+        for st in range(G):
+            for i in range(n if ipa is None else ipa + 1):
+                ad = [2 * ((g * G + st) * n + i) + h for g in range(ng) for h in (0, 1)]
+                worst = max(worst, _lds_conflict(ad))
+            if ipa is not None:
+                ad = [2 * ((g * G + st) * n + ipa + k) + h for g in range(ng) for k in range(4) for h in (0, 1)]
+                worst = max(worst, _lds_conflict(ad))
+        if best is None or worst < best[0]:
+            best = (worst, n)
+    return best[1]
 
 
 def kernel_name(cls, spherical, prec="d"):
     return "intc_eri_%s%d%d%d%d_%s" % (prec, *cls, "s" if spherical else "c")
 
 
-def _chain_lines(LX, M, R="double"):
-    """Bra chain: c{s}_{m} for s = 0..LX, m = 0..M-s; c0 given (type R)."""
+def _chain_lines(LX, M, R="double", slot_pw=None):
+    """Bra chain: c{s}_{m} for s = 0..LX, m = 0..M-s; c0 given (type R).
+
+    slot_pw = (iPA, iWP): the setup slot u holds PA_x, PA_y, PA_z, 1 at iPA
+    and WP_x, WP_y, WP_z, 0 at iWP, and each lane reads its step direction's
+    entry directly instead of selecting among the three components."""
     one = "1.0f" if R == "float" else "1.0"
     zero = "0.0f" if R == "float" else "0.0"
     L = []
     for s in range(1, LX + 1):
         L.append("const int d%d = (cdir >> %d) & 3;" % (s, 2 * (s - 1)))
-        L.append("const %s pa%d = d%d == 0 ? PAx : (d%d == 1 ? PAy : (d%d == 2 ? PAz : %s));"
-                 % (R, s, s, s, s, one))
-        L.append("const %s wp%d = d%d == 0 ? WPx : (d%d == 1 ? WPy : (d%d == 2 ? WPz : %s));"
-                 % (R, s, s, s, s, zero))
+        if slot_pw:
+            L.append("const %s pa%d = (%s)u[%d + d%d];" % (R, s, R, slot_pw[0], s))
+            L.append("const %s wp%d = (%s)u[%d + d%d];" % (R, s, R, slot_pw[1], s))
+        else:
+            L.append("const %s pa%d = d%d == 0 ? PAx : (d%d == 1 ? PAy : (d%d == 2 ? PAz : %s));"
+                     % (R, s, s, s, s, one))
+            L.append("const %s wp%d = d%d == 0 ? WPx : (d%d == 1 ? WPy : (d%d == 2 ? WPz : %s));"
+                     % (R, s, s, s, s, zero))
         if s >= 2:
             L.append("const %s kk%d = (%s)((ckk >> %d) & 15) * o2z;" % (R, s, R, 4 * (s - 1)))
         for m in range(M - s + 1):
@@ -273,14 +315,17 @@ def emit_class_source(cls, spherical, prec="d", waves_per_eu=None, batch=None):
     for m in range(M + 1):
         slot.append(("c0_%d" % m, R))
     if LX > 0:
-        slot += [(n, R) for n in ("PAx", "PAy", "PAz", "WPx", "WPy", "WPz", "roz")] + [("o2z", "double")]
+        # PA and WP each padded with the value of the 'no step' direction (3)
+        slot += [(n, R) for n in ("PAx", "PAy", "PAz", "ONE", "WPx", "WPy", "WPz", "ZERO", "roz")]
+        slot += [("o2z", "double")]
     if LR > 0:
         slot += [(n, R) for n in ("QCx", "QCy", "QCz", "WQx", "WQy", "WQz", "roe")] + [("o2e", "double")]
         if LX > 0:
             slot.append(("o2s", R))
     if prec == "f":
         slot.append(("pref", "double"))
-    NS = len(slot) if batched else 0
+    names = [nm for nm, _ in slot]
+    NS = slot_stride(G, len(slot), names.index("PAx") if LX > 0 else None) if batched else 0
     o("constexpr int G = %d, NE = %d, NCD = %d, M = %d, NS = %d;" % (G, NE, NCD, M, NS))
     o("extern __shared__ double lds[];")
     o("const int T = a.team_groups, TL = G * T, TPW = 64 / TL;")
@@ -364,12 +409,12 @@ def emit_class_source(cls, spherical, prec="d", waves_per_eu=None, batch=None):
             for m in range(M + 1):
                 o("const double c0_%d = pref * F[%d];" % (m, m))
 
-    def emit_recurrences():
+    def emit_recurrences(slot_pw=None):
         """bra chain + ket VRR + accumulation on the names of the setup"""
         if LX > 0 and LR > 0:
             o("const %s cplx = ex_d * o2s, cply = ey_d * o2s, cplz = ez_d * o2s;" % R)
         if prec == "f":
-            body = _chain_lines(LX, M, R)
+            body = _chain_lines(LX, M, R, slot_pw)
             lines, V = _ket_lines(LX, LR, R)
             body += lines
             body = [ln.replace("o2z", "o2zf").replace("o2e", "o2ef") for ln in body]
@@ -378,7 +423,7 @@ def emit_class_source(cls, spherical, prec="d", waves_per_eu=None, batch=None):
             for f in acc_f:
                 o("acc%d = fma(pref, (double)%s, acc%d);" % (cidx(f), V(f, 0), cidx(f)))
         else:
-            for ln in _chain_lines(LX, M):
+            for ln in _chain_lines(LX, M, "double", slot_pw):
                 o(ln)
             lines, V = _ket_lines(LX, LR)
             for ln in lines:
@@ -405,7 +450,8 @@ def emit_class_source(cls, spherical, prec="d", waves_per_eu=None, batch=None):
         o("const int ib = pq / Pk, ik = pq - ib * Pk;")
         emit_setup(False)
         for i, (nm, ty) in enumerate(slot):
-            o("myslot[%d] = (double)%s;" % (i, nm))
+            val = {"ONE": "1.0", "ZERO": "0.0"}.get(nm, "(double)" + nm)
+            o("myslot[%d] = %s;" % (i, val))
         o.ind = 3
         o("}")
         o("__builtin_amdgcn_fence(__ATOMIC_SEQ_CST, \"wavefront\");")
@@ -414,7 +460,11 @@ def emit_class_source(cls, spherical, prec="d", waves_per_eu=None, batch=None):
         o("for (int st = 0; st < ns; ++st) {")
         o.ind = 4
         o("const double *u = gslot + st * NS;")
+        # PA / WP are read per lane by the bra chain (its step direction)
+        lane_read = {"PAx", "PAy", "PAz", "ONE", "WPx", "WPy", "WPz", "ZERO"} if SLOT_PW else {"ONE", "ZERO"}
         for i, (nm, ty) in enumerate(slot):
+            if nm in lane_read:
+                continue
             if ty == "double":
                 o("const double %s = u[%d];" % (nm, i))
             else:
@@ -424,7 +474,8 @@ def emit_class_source(cls, spherical, prec="d", waves_per_eu=None, batch=None):
                 o("const float o2zf = (float)o2z;")
             if LR > 0:
                 o("const float o2ef = (float)o2e;")
-        emit_recurrences()
+        names = [nm for nm, _ in slot]
+        emit_recurrences((names.index("PAx"), names.index("WPx")) if LX > 0 and SLOT_PW else None)
         o.ind = 3
         o("}")
         o("__builtin_amdgcn_fence(__ATOMIC_SEQ_CST, \"wavefront\");")

@@ -200,8 +200,9 @@ __global__ void __launch_bounds__(256) intc_mlp_route_k(long long n, const float
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         __builtin_amdgcn_wave_barrier();
         float mine = 0.0f;  // network output of quartet 64 t + lane
+        // all four tiles always: rows past n hold the features of quartet
+        // n - 1 and are not stored
         for (int tt = 0; tt < 4; ++tt) {
-            if (t * 64 + 16 * tt >= n) break;  // uniform over the wave
             // B of layer 1: X^T, features 4 g4 + j of quartet 16 tt + r16
             bf16x4 bx;
             for (int j = 0; j < 4; ++j) bx[j] = (short)f2bf(sF[wave][16 * tt + r16][4 * g4 + j]);

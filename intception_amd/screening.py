@@ -162,7 +162,9 @@ def build_router(n_train=2_000_000, train_seeds=(202611, 202612), val_seed=20261
              "val_max_underprediction": float(np.max(tv - pred)), "margin": margin,
              "val_fp32_max_err": float(e32.max()), "val_fp32_fraction": n32 / len(pred),
              "val_samples": int(len(pred)),
-             "val_skip_fraction": float(np.mean(pred + margin < np.log10(tau_skip))), **router.summary()}
+             "val_skip_fraction": float(np.mean(pred + margin < np.log10(tau_skip))),
+             # what a perfect predictor could skip: true magnitude below tau_skip
+             "val_skip_ideal": float(np.mean(tv < np.log10(tau_skip))), **router.summary()}
     if log:
         log("router: %s" % {k: v for k, v in stats.items() if k != "train_mse"})
     return router, stats

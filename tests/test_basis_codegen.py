@@ -78,6 +78,30 @@ def test_generated_kernel_is_well_formed():
     assert meta["G"] == 20 and meta["NCD"] == 25
 
 
+def test_slot_stride_avoids_bank_conflicts():
+    # brute force over the wave's slot reads: the chosen stride is >= the
+    # slot length, no worse than any stride up to 8 longer, and for 4-lane
+    # groups never a multiple of 8 doubles (16 groups on the same banks)
+    from intception_amd.codegen import _lds_conflict, slot_stride
+
+    def worst(G, n, ipa):
+        w = 0
+        for st in range(G):
+            for i in range(n):
+                w = max(w, _lds_conflict([2 * ((g * G + st) * n + i) + h for g in range(64 // G) for h in (0, 1)]))
+            w = max(w, _lds_conflict([2 * ((g * G + st) * n + ipa + k) + h
+                                      for g in range(64 // G) for k in range(4) for h in (0, 1)]))
+        return w
+    for G in (4, 10, 20):
+        for ns in range(18, 30):
+            n = slot_stride(G, ns, 5)
+            assert ns <= n <= ns + 8
+            assert worst(G, n, 5) <= min(worst(G, m, 5) for m in range(ns, ns + 9))
+            if G == 4:
+                assert n % 8 != 0 and worst(G, n, 5) <= 2
+    assert _lds_conflict([0, 64, 128, 1]) == 3
+
+
 def test_flops_model_small_classes():
     # (ps|ss): a-build of 3 components at m = 0, 2 terms each
     assert flops_model.vrr_terms(1, 0, 0, 0) == 3 * 2
