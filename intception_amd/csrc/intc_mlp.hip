@@ -144,6 +144,9 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
 __device__ __forceinline__ float bf16_round(float x) { return (float)(__bf16)x; }
 
+// max(x, 0) on the bit pattern: one integer max, no NaN canonicalisation
+__device__ __forceinline__ float relu_bits(float x) { return __int_as_float(max(__float_as_int(x), 0)); }
+
 __global__ void __launch_bounds__(256) intc_mlp_route_k(long long n, const float *__restrict__ pf,
                                                         const int *__restrict__ qp, const float *__restrict__ norm,
                                                         const uint16_t *__restrict__ wb,
@@ -158,7 +161,8 @@ __global__ void __launch_bounds__(256) intc_mlp_route_k(long long n, const float
     // W3 entries of the hidden units this lane's accumulator slots hold
     bf16x4 a1w[4];
     bf16x8 a2w[4][2];
-    float b1r[4][4], b2r[4][4], w3r[4][4];
+    f32x4 b1v[4], b2v[4];
+    float w3r[4][4];
     for (int nt = 0; nt < 4; ++nt) {
         const int row = 16 * nt + r16;
         for (int j = 0; j < 4; ++j) a1w[nt][j] = (short)wb[BW1 + row * KF + 4 * g4 + j];
@@ -167,8 +171,8 @@ __global__ void __launch_bounds__(256) intc_mlp_route_k(long long n, const float
                 a2w[nt][c][sl] = __builtin_bit_cast(__bf16, wb[BW2 + row * NH + 16 * (2 * c + sl / 4) + 4 * g4 + sl % 4]);
         for (int j = 0; j < 4; ++j) {
             const int h = 16 * nt + 4 * g4 + j;
-            b1r[nt][j] = master[OB1 + h];
-            b2r[nt][j] = master[OB2 + h];
+            b1v[nt][j] = master[OB1 + h];
+            b2v[nt][j] = master[OB2 + h];
             w3r[nt][j] = bf2f(wb[BW3 + h]);
         }
     }
@@ -205,25 +209,24 @@ __global__ void __launch_bounds__(256) intc_mlp_route_k(long long n, const float
         for (int tt = 0; tt < 4; ++tt) {
             // B of layer 1: X^T, features 4 g4 + j of quartet 16 tt + r16
             bf16x4 bx;
-            for (int j = 0; j < 4; ++j) bx[j] = (short)f2bf(sF[wave][16 * tt + r16][4 * g4 + j]);
+            for (int j = 0; j < 4; ++j) bx[j] = __builtin_bit_cast(short, (__bf16)sF[wave][16 * tt + r16][4 * g4 + j]);
             f32x4 d1[4];
             for (int nt = 0; nt < 4; ++nt)
-                d1[nt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a1w[nt], bx, f32x4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+                d1[nt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a1w[nt], bx, b1v[nt], 0, 0, 0);
             // B of layer 2: relu(H1) in BF16, K chunk c slot s = unit 16 (2c + s/4) + 4 g4 + s%4
             bf16x8 h1[2];
             for (int c = 0; c < 2; ++c)
                 for (int sl = 0; sl < 8; ++sl) {
-                    const float h = d1[2 * c + sl / 4][sl % 4] + b1r[2 * c + sl / 4][sl % 4];
-                    h1[c][sl] = (__bf16)(h > 0.f ? h : 0.f);
+                    const float h = d1[2 * c + sl / 4][sl % 4];
+                    h1[c][sl] = (__bf16)relu_bits(h);
                 }
             float part = 0.0f;
             for (int nt = 0; nt < 4; ++nt) {
-                f32x4 d2 = {0.f, 0.f, 0.f, 0.f};
-                d2 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2w[nt][0], h1[0], d2, 0, 0, 0);
+                f32x4 d2 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2w[nt][0], h1[0], b2v[nt], 0, 0, 0);
                 d2 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2w[nt][1], h1[1], d2, 0, 0, 0);
                 for (int j = 0; j < 4; ++j) {
-                    const float h = d2[j] + b2r[nt][j];
-                    part += bf16_round(h > 0.f ? h : 0.f) * w3r[nt][j];
+                    const float h = d2[j];
+                    part += bf16_round(relu_bits(h)) * w3r[nt][j];
                 }
             }
             part += __shfl_xor(part, 16);

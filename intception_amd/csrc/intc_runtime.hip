@@ -889,7 +889,7 @@ INTC_API int intc_plan_run_bin(intc_plan *p, int i, int prec, const double *sph,
 }
 
 // device route: one int8 per caller quartet (0 FP64, 1 FP32, 2 skip)
-static int route_queues(intc_plan *p, const signed char *route, double *out, hipStream_t st)
+static int route_queues(intc_plan *p, const signed char *route, hipStream_t st)
 {
     int nb = (int)p->bins.size();
     if (p->n_quartet == 0) return 0;
@@ -903,6 +903,14 @@ static int route_queues(intc_plan *p, const signed char *route, double *out, hip
     hipLaunchKernelGGL(intc_route_scatter, dim3(p->n_tile), dim3(256), 0, st, route, p->q_orig, ident, p->tile_q0,
                        p->tile_n, p->tile_bin, p->bin_q0, p->tile_base, use32, p->idx64, p->idx32, p->idxs);
     HIPCHK(hipGetLastError());
+    return 0;
+}
+
+// zeros the skipped blocks (after route_queues on the same or a joined stream)
+static int zero_skipped(intc_plan *p, double *out, hipStream_t st)
+{
+    const int nb = (int)p->bins.size();
+    if (p->n_quartet == 0 || nb == 0) return 0;
     hipLaunchKernelGGL(intc_zero_skipped, dim3(1024, nb), dim3(256), 0, st, p->bin_q0, p->counts, p->idxs,
                        p->q_off, p->q_size, out);
     HIPCHK(hipGetLastError());
@@ -916,13 +924,20 @@ static int run_all(intc_plan *p, const double *sph, double *out, const signed ch
     int rc = fill_args(p, sph, out, &a);
     if (rc) return rc;
     if (route) {
-        rc = route_queues(p, route, out, st);
+        rc = route_queues(p, route, st);
         if (rc) return rc;
     }
     if (n_streams > 1) {
         if (hipEventRecord(p->events[0], st) != hipSuccess) return fail(-3, "event record");
         for (int k = 0; k < n_streams - 1; ++k)
             if (hipStreamWaitEvent(p->side[k], p->events[0], 0) != hipSuccess) return fail(-3, "stream wait");
+    }
+    // Skipped blocks are disjoint from every block a class kernel writes, so
+    // the HBM-bound zeroing runs beside the arithmetic-bound bins: first on the
+    // last stream (the stream that receives the fewest bins).
+    if (route) {
+        rc = zero_skipped(p, out, n_streams > 1 ? p->side[n_streams - 2] : st);
+        if (rc) return rc;
     }
     // largest bins first, round-robin over the streams
     std::vector<int> ord(p->bins.size());
