@@ -173,7 +173,8 @@ class MagnitudeNet:
     def __init__(self, device="cuda", seed=0):
         import torch
         self.torch = torch
-        self.device = torch.device(device)
+        from .runtime import _cuda_device
+        self.device = _cuda_device(torch, device)
         rng = np.random.default_rng(seed)
         m = np.zeros(NPARAM, dtype=np.float32)
         m[OW1:OB1] = (rng.normal(0, np.sqrt(2.0 / NF), size=(NH, KF)) * (np.arange(KF) < NF)).reshape(-1)

@@ -30,6 +30,15 @@ def sph_blob(lmax=7):
     return np.concatenate(parts)
 
 
+def _cuda_device(torch, device):
+    """torch.device with an explicit index ("cuda" -> the current device),
+    which initialises torch's CUDA state before any device guard uses it."""
+    d = torch.device(device)
+    if d.type == "cuda" and d.index is None:
+        d = torch.device("cuda", torch.cuda.current_device())
+    return d
+
+
 def _ptr(t):
     return ctypes.c_void_p(t.data_ptr())
 
@@ -42,7 +51,7 @@ class EriPlan:
         torch = _torch()
         L = _lib.load()
         self.batch = batch
-        self.device = torch.device(device)
+        self.device = _cuda_device(torch, device)
         self._L = L
         b = batch
         sl = np.ascontiguousarray(b.shell_l, dtype=np.int32)
