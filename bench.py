@@ -169,10 +169,12 @@ def run_molecule(args, three_center=False):
     log("plain FP64: %.3f ms/step" % ms64)
     sizes = torch.from_numpy(batch.sizes()).to(dev)
 
-    def run_learned(use_fp32):
+    def run_learned(use_fp32, persistent=False):
         """router kernel (features + MFMA network + route codes) then the
-        routed graph (pair builder, route compaction, FP64 [and FP32] queues)"""
+        routed graph (pair builder, route compaction, FP64 [and FP32] queues;
+        with persistent=True every queue runs in the one multi-class launch)"""
         plan.set_route_fp32(use_fp32)
+        plan.set_persistent(persistent)
         plan.capture(outml, n_streams=args.streams, route=route)
 
         def step():
@@ -210,6 +212,25 @@ def run_molecule(args, three_center=False):
     log("router kernel: %.3f ms" % ms_route)
     learned_skip = run_learned(False)
     learned_mixed = run_learned(True)
+    # the same two paths as ONE persistent multi-class launch (tiles of every
+    # class bin from a device counter), captured in a graph
+    plan.set_persistent(True)
+    plan.capture(outml, n_streams=args.streams)
+    for _ in range(args.warmup):
+        plan.replay()
+    ms_p64 = _timed(torch, dev, stream, plan.replay, args.steps)
+    same = bool(torch.equal(outml, out64))
+    n_tiles = plan.persistent_info()[0]
+    log("plain FP64, one persistent launch: %.3f ms/step (output identical to the per-class graph: %s)"
+        % (ms_p64, same))
+    learned_p = run_learned(True, persistent=True)
+    plan.set_persistent(False)
+    persistent = {"plain_fp64_ms_per_step": float(ms_p64), "plain_output_identical": same,
+                  "learned_skip_and_fp32_ms_per_step": learned_p["ms_per_step"],
+                  "learned_max_abs_err_vs_fp64": learned_p["max_abs_err_vs_fp64"],
+                  "tile_iters": 64, "grid_blocks": 512,
+                  "note": "one launch of the multi-class kernel (every spherical register class) per step; "
+                          "its occupancy is the worst class's (2 waves/SIMD), so it trails the per-class graph"}
     ok = [v for v in (learned_mixed, learned_skip) if v["meets_1e-10"]]
     # the reported value comes from a learned variant within the error bar; if
     # neither is, the plain FP64 leg is the valid result and is reported
@@ -255,6 +276,7 @@ def run_molecule(args, three_center=False):
                        "quartets_per_s": float(batch.n_quartet / (ms64 * 1e-3))},
         "learned_skip_fp64": learned_skip,
         "learned_skip_and_fp32": learned_mixed,
+        "persistent_launch": persistent,
         "value_from": ("learned_skip_and_fp32" if best is learned_mixed else
                        "learned_skip_fp64" if best is learned_skip else
                        "plain_fp64 (no learned variant within 1e-10)"),
@@ -325,6 +347,21 @@ def run_config1(args):
         log("autotune: %.1f s" % (time.perf_counter() - t0))
     ms_eager = timed(lambda: plan.run(out), 2)
     log("one launch per class, tuned shapes: %.3f ms/step" % ms_eager)
+    ref_tuned = out.clone()
+    # one persistent multi-class launch per step, tile size swept
+    sweep = {}
+    for ti in (16, 64, 256):
+        plan.set_persistent(True, ti)
+        plan.run(out)
+        sweep[ti] = timed(lambda: plan.run(out), 2)
+        log("one persistent launch, tile_iters %d: %.3f ms/step (%d tiles)" % (ti, sweep[ti],
+                                                                               plan.persistent_info()[0]))
+    best_ti = min(sweep, key=sweep.get)
+    plan.set_persistent(True, best_ti)
+    plan.run(out)
+    same = bool(torch.equal(out, ref_tuned))
+    plan.set_persistent(False)
+    del ref_tuned
     plan.capture(out, n_streams=args.streams)
     for _ in range(args.warmup):
         plan.replay()
@@ -421,6 +458,9 @@ def run_config1(args):
         "plan_s": t_plan,
         "ms_one_launch_per_class_heuristic": ms_default,
         "ms_one_launch_per_class_tuned": ms_eager,
+        "ms_one_persistent_launch": sweep[best_ti],
+        "persistent_tile_iters_ms": {str(k): v for k, v in sweep.items()},
+        "persistent_output_identical": same,
         "graph_streams": args.streams,
         "pair_build_ms": t_pairs,
         "bins": [{"cls": r["cls"], "nq": r["nq"], "T": r["team_groups"], "ms": round(r["ms"], 4),

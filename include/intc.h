@@ -126,6 +126,18 @@ int intc_plan_graph_launch(intc_plan *plan, void *stream);
  * route code 1 as FP64 and launches the FP64 kernels only (no effect on
  * graphs already captured). */
 int intc_plan_set_route_fp32(intc_plan *plan, int use_fp32);
+/* Persistent multi-class launch (no counterpart in the reference): with
+ * enable = 1, execute, execute_routed and graphs captured afterwards run every
+ * bin whose class the multi-class kernel holds (all spherical register
+ * classes, FP64 and FP32 recurrences) in ONE launch: waves take tiles of
+ * (teams per wave) x (at most tile_iters) quartet positions of one bin from a
+ * device counter, bins interleaved, and run that class's code with the bin's
+ * team size; a tile closes early at 1/8 of an average wave's estimated share.  Other bins still get their own kernels.  Spherical plans
+ * with the built-in transforms (sph = NULL) and Boys mode 0 only (-2 / -1
+ * otherwise).  grid: blocks of 256 threads, 0 = two per CU. */
+int intc_plan_set_persistent(intc_plan *plan, int enable, int tile_iters, int grid);
+/* Tiles and LDS doubles per wave of the last persistent eager execution. */
+int intc_plan_persistent_info(const intc_plan *plan, int *n_tiles, int *lds_doubles_per_wave);
 /* FP64/FP32/skip queue lengths per bin of the last routed execution (3 ints per bin). */
 int intc_plan_route_counts(intc_plan *plan, int *counts);
 /* Plan order: order[i] = caller's index of the quartet at plan position i

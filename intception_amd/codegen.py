@@ -293,18 +293,6 @@ def emit_class_source(cls, spherical, prec="d", waves_per_eu=None, batch=None):
         LA, LB, LC, LD, "spherical" if spherical else "Cartesian", "FP64" if prec == "d" else "FP32"))
     o('#include "../intc_common.h"')
     o("")
-    o("__constant__ int %s_lane[%d][8] = {" % (name, G))
-    for r in lane_tab:
-        o("    {%s}," % ", ".join(str(v) for v in r))
-    o("};")
-    o("")
-    # BM: Boys scheme (0 Chebyshev, 1 reference); BS: 1 = the built-in real
-    # solid-harmonic matrices are compiled in as literals (zero terms dropped),
-    # 0 = the caller's matrices are read from a.sph
-    o("template <int BM, int BS>")
-    o("__device__ __forceinline__ void %s_body(const IntcEriArgs &a)" % name)
-    o("{")
-    o.ind = 1
     # Batched setup (groups of G >= BATCH_MIN_G lanes): lane e of a group
     # computes the Boys values and recurrence coefficients of the group's
     # primitive quartet kb + e into its LDS slot, then the group walks the G
@@ -326,16 +314,41 @@ def emit_class_source(cls, spherical, prec="d", waves_per_eu=None, batch=None):
         slot.append(("pref", "double"))
     names = [nm for nm, _ in slot]
     NS = slot_stride(G, len(slot), names.index("PAx") if LX > 0 else None) if batched else 0
+
+    # The body is shared by the per-class kernels below and by the persistent
+    # multi-class kernel, which includes this file with INTC_BODIES_ONLY
+    # defined (file-scope names have internal linkage for that reason).
+    o("static __constant__ int %s_lane[%d][8] = {" % (name, G))
+    for r in lane_tab:
+        o("    {%s}," % ", ".join(str(v) for v in r))
+    o("};")
+    o("// G lanes per group, NE kept bra rows, NCD ket columns, NS setup-slot length")
+    o("static constexpr int %s_G = %d, %s_NE = %d, %s_NCD = %d, %s_NS = %d;" % (name, G, name, NE, name, NCD,
+                                                                                name, NS))
+    o("// LDS doubles per wave for team size T: max(epilogue transpose of its teams, 64 setup slots)")
+    o("__host__ __device__ constexpr int %s_reg(int T)" % name)
+    o("{")
+    o("    return (64 / (%d * T)) * %d > 64 * %d ? (64 / (%d * T)) * %d : 64 * %d;" % (G, NE * NCD, NS, G, NE * NCD,
+                                                                                     NS))
+    o("}")
+    o("")
+    # BM: Boys scheme (0 Chebyshev, 1 reference); BS: 1 = the built-in real
+    # solid-harmonic matrices are compiled in as literals (zero terms dropped),
+    # 0 = the caller's matrices are read from a.sph.  The calling wave works on
+    # queue positions qq0 + team, qq0 + team + qstep, ... below qend (and below
+    # the bin's length); wlds is its LDS region of NAME_reg(T) doubles.
+    o("template <int BM, int BS>")
+    o("__device__ __forceinline__ void %s_body(const IntcEriArgs &a, double *wlds, const int qq0, const int qstep,"
+      " int qend)" % name)
+    o("{")
+    o.ind = 1
     o("constexpr int G = %d, NE = %d, NCD = %d, M = %d, NS = %d;" % (G, NE, NCD, M, NS))
-    o("extern __shared__ double lds[];")
     o("const int T = a.team_groups, TL = G * T, TPW = 64 / TL;")
-    o("const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;")
+    o("const int lane = threadIdx.x & 63;")
     o("const int tw = lane / TL, tl = lane - tw * TL;")
     o("const int grp = tl / G, e = tl - grp * G;")
     o("if (tw >= TPW) return;")
     o("// per-wave LDS region: the setup slots (batched loop), then the epilogue transpose")
-    o("const int REG = (TPW * NE * NCD > 64 * NS) ? TPW * NE * NCD : 64 * NS;")
-    o("double *wlds = lds + (size_t)wave * REG;")
     o("double *tlds = wlds + (size_t)tw * (NE * NCD);")
     o("const int *lt = %s_lane[e];" % name)
     o("const %s ex_d = (%s)lt[0], ey_d = (%s)lt[1], ez_d = (%s)lt[2];" % (R, R, R, R))
@@ -343,9 +356,9 @@ def emit_class_source(cls, spherical, prec="d", waves_per_eu=None, batch=None):
     o("const int s4x = (lbase + lt[3]) * 4, s4y = (lbase + lt[4]) * 4, s4z = (lbase + lt[5]) * 4;")
     o("const int cdir = lt[6], ckk = lt[7];")
     o("(void)ex_d; (void)ey_d; (void)ez_d; (void)s4x; (void)s4y; (void)s4z; (void)cdir; (void)ckk;")
-    o("const int nteams = gridDim.x * (blockDim.x >> 6) * TPW;")
     o("const int nq = a.nq_dev ? *a.nq_dev : a.nq;")
-    o("for (int qq = (blockIdx.x * (blockDim.x >> 6) + wave) * TPW + tw; qq < nq; qq += nteams) {")
+    o("if (qend > nq) qend = nq;")
+    o("for (int qq = qq0 + tw; qq < qend; qq += qstep) {")
     o.ind = 2
     o("const int q = a.q_idx ? a.q_idx[qq] : qq;")
     o("if (q < 0 || q >= a.nq) continue;  // queue entries index the bin")
@@ -594,24 +607,31 @@ def emit_class_source(cls, spherical, prec="d", waves_per_eu=None, batch=None):
     o.ind = 0
     o("}")
     o("")
+    o("#ifndef INTC_BODIES_ONLY")
+    # one-class grid: wave w of block b starts at team index (b * waves + w) * TPW
+    # and strides over every team of the grid
+    o("template <int BM, int BS>")
+    o("__device__ __forceinline__ void %s_grid(const IntcEriArgs &a)" % name)
+    o("{")
+    o("    extern __shared__ double lds[];")
+    o("    const int tpw = 64 / (%d * a.team_groups), wave = threadIdx.x >> 6;" % G)
+    o("    %s_body<BM, BS>(a, lds + (size_t)wave * %s_reg(a.team_groups), (blockIdx.x * (blockDim.x >> 6) + wave)"
+      " * tpw, gridDim.x * (blockDim.x >> 6) * tpw, a.nq);" % (name, name))
+    o("}")
     # waves_per_eu: occupancy target for the Chebyshev-mode kernel (the compiler
     # then caps its registers at 512 / waves VGPRs)
     occ = " __attribute__((amdgpu_waves_per_eu(%d)))" % waves_per_eu if waves_per_eu else ""
-    o('extern "C" __global__ void __launch_bounds__(256)%s %s(const IntcEriArgs a) { %s_body<0, 0>(a); }'
+    o('extern "C" __global__ void __launch_bounds__(256)%s %s(const IntcEriArgs a) { %s_grid<0, 0>(a); }'
       % (occ, name, name))
-    o('extern "C" __global__ void __launch_bounds__(256) %s_refboys(const IntcEriArgs a) { %s_body<1, 0>(a); }'
+    o('extern "C" __global__ void __launch_bounds__(256) %s_refboys(const IntcEriArgs a) { %s_grid<1, 0>(a); }'
       % (name, name))
     if spherical:
-        o('extern "C" __global__ void __launch_bounds__(256)%s %s_bs(const IntcEriArgs a) { %s_body<0, 1>(a); }'
+        o('extern "C" __global__ void __launch_bounds__(256)%s %s_bs(const IntcEriArgs a) { %s_grid<0, 1>(a); }'
           % (occ, name, name))
     o('extern "C" hipError_t %s_launch(const IntcEriArgs *a, int grid, int block, size_t lds, hipStream_t st)'
       % name)
     o("{")
-    o("    // per wave: max(epilogue transpose of its teams, 64 setup slots)")
-    o("    (void)lds;")
-    o("    const int tpw = 64 / (%d * a->team_groups);" % G)
-    o("    const size_t reg = (size_t)((tpw * %d > 64 * %d) ? tpw * %d : 64 * %d);" % (NE * NCD, NS, NE * NCD, NS))
-    o("    lds = (size_t)(block / 64) * reg * sizeof(double);")
+    o("    lds = (size_t)(block / 64) * %s_reg(a->team_groups) * sizeof(double);" % name)
     if spherical:
         o("    if (a->boys_mode == 0 && a->sph_builtin)")
         o("        hipLaunchKernelGGL(%s_bs, dim3(grid), dim3(block), lds, st, *a);" % name)
@@ -623,7 +643,8 @@ def emit_class_source(cls, spherical, prec="d", waves_per_eu=None, batch=None):
     o("        hipLaunchKernelGGL(%s_refboys, dim3(grid), dim3(block), lds, st, *a);" % name)
     o("    return hipGetLastError();")
     o("}")
-    meta = dict(cls=cls, spherical=spherical, G=G, NE=NE, NCD=NCD, M=M, name=name, prec=prec)
+    o("#endif  // INTC_BODIES_ONLY")
+    meta = dict(cls=cls, spherical=spherical, G=G, NE=NE, NCD=NCD, M=M, NS=NS, name=name, prec=prec)
     return o.text() + "\n", meta
 
 
@@ -675,6 +696,88 @@ def _nout(l, sph):
     return 2 * l + 1 if sph else (l + 1) * (l + 2) // 2
 
 
+def multi_members(metas):
+    """Class kernels compiled into the persistent multi-class kernel: every
+    spherical register kernel, FP64 and FP32 recurrences, except the FP32
+    kernels of classes with an f or g shell (switched off in the runtime,
+    which gives those bins the FP64 kernel)."""
+    out = []
+    for m in metas:
+        if not m["spherical"]:
+            continue
+        if m["prec"] == "f" and max(m["cls"]) >= 3:
+            continue
+        out.append(m)
+    return out
+
+
+def emit_multi_source(members):
+    """The persistent multi-class kernel: waves take tiles (one bin, a range
+    of its quartet positions) from a device counter and run that class's body
+    on them with its team size, until the tiles run out."""
+    L = ["// Generated by intception_amd/codegen.py -- do not edit.",
+         "// Persistent multi-class kernel: every spherical register-class body (built-in",
+         "// transforms, Chebyshev Boys) behind one tile queue.",
+         "#define INTC_BODIES_ONLY"]
+    for m in members:
+        L.append('#include "%s.hip"' % m["name"])
+    L += ["#undef INTC_BODIES_ONLY",
+          "",
+          "// tiles[t] = {bin, kind, q_lo, q_hi}: queue positions [q_lo, q_hi) of bin `bin`,",
+          "// whose body is case `kind`; args[bin] holds that bin's kernel arguments.",
+          "// `next` must be 0 at launch (intc_multi_reset); every wave stops once the",
+          "// counter has passed n_tiles.  reg: LDS doubles per wave (max over the bins).",
+          "// Two waves per SIMD: the largest bodies need up to 256 VGPRs (a few spill).",
+          'extern "C" __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) '
+          "intc_eri_multi_s_bs(const IntcEriArgs *args, const int4 *tiles, int n_tiles, int *next, int reg)",
+          "{",
+          "    extern __shared__ double lds[];",
+          "    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;",
+          "    double *wlds = lds + (size_t)wave * reg;",
+          "    for (;;) {",
+          "        int t = 0;",
+          "        if (lane == 0) t = atomicAdd(next, 1);",
+          "        t = __builtin_amdgcn_readfirstlane(__shfl(t, 0));",
+          "        if (t >= n_tiles) break;",
+          "        const int4 w = tiles[t];",
+          "        const int bin = __builtin_amdgcn_readfirstlane(w.x), kind = __builtin_amdgcn_readfirstlane(w.y);",
+          "        const int q_lo = __builtin_amdgcn_readfirstlane(w.z), q_hi = __builtin_amdgcn_readfirstlane(w.w);",
+          "        const IntcEriArgs &a = args[bin];",
+          "        switch (kind) {"]
+    for k, m in enumerate(members):
+        L.append("        case %d: %s_body<0, 1>(a, wlds, q_lo, 64 / (%d * a.team_groups), q_hi); break;"
+                 % (k, m["name"], m["G"]))
+    L += ["        default: break;",
+          "        }",
+          '        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");',
+          "        __builtin_amdgcn_wave_barrier();",
+          "    }",
+          "}",
+          "",
+          'extern "C" __global__ void intc_multi_reset(int *next) { if (threadIdx.x == 0) *next = 0; }',
+          "",
+          "// LDS beyond 64 KB per block needs the attribute on this device",
+          'extern "C" int intc_multi_prepare()',
+          "{",
+          "    return hipFuncSetAttribute((const void *)intc_eri_multi_s_bs, "
+          "hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024) == hipSuccess ? 0 : -1;",
+          "}",
+          "",
+          "// counter reset (a kernel, so that graphs hold no memset node), then the persistent grid",
+          'extern "C" hipError_t intc_multi_launch(const IntcEriArgs *args, const int4 *tiles, int n_tiles, '
+          "int *next, int reg, int grid, hipStream_t st)",
+          "{",
+          "    hipLaunchKernelGGL(intc_multi_reset, dim3(1), dim3(64), 0, st, next);",
+          "    hipError_t e = hipGetLastError();",
+          "    if (e != hipSuccess) return e;",
+          "    hipLaunchKernelGGL(intc_eri_multi_s_bs, dim3(grid), dim3(256), (size_t)4 * reg * sizeof(double), st, "
+          "args, tiles, n_tiles, next, reg);",
+          "    return hipGetLastError();",
+          "}",
+          ""]
+    return "\n".join(L)
+
+
 def generate(outdir, classes, sph_modes=(False, True), precs=("d", "f"), wg=(), occupancy=None, batch=None):
     os.makedirs(outdir, exist_ok=True)
     metas = []
@@ -702,19 +805,25 @@ def generate(outdir, classes, sph_modes=(False, True), precs=("d", "f"), wg=(), 
                          'hipStream_t);' % pr)
     # G = 64 and one group per team for the workgroup kernels (wg = 1): one
     # quartet per 256-thread block, LDS sized by the launcher
-    lines.append("struct IntcClassEntry { int la, lb, lc, ld, sph, prec, G, NE, NCD, M, wg; intc_launch_fn fn; };")
+    # NS: setup-slot length (LDS per wave is max(teams per wave * NE * NCD,
+    # 64 * NS) doubles); multi: case of the class in intc_eri_multi_s_bs, -1
+    # when the persistent kernel does not hold it
+    lines.append("struct IntcClassEntry { int la, lb, lc, ld, sph, prec, G, NE, NCD, M, wg, NS, multi; "
+                 "intc_launch_fn fn; };")
     lines.append("static const IntcClassEntry intc_class_table[] = {")
+    members = multi_members(metas)
+    kind = {m["name"]: k for k, m in enumerate(members)}
     n = 0
     for m in metas:
         la, lb, lc, ld = m["cls"]
-        lines.append("    {%d, %d, %d, %d, %d, %d, %d, %d, %d, %d, 0, %s_launch}," % (
+        lines.append("    {%d, %d, %d, %d, %d, %d, %d, %d, %d, %d, 0, %d, %d, %s_launch}," % (
             la, lb, lc, ld, int(m["spherical"]), 0 if m["prec"] == "d" else 1, m["G"], m["NE"], m["NCD"], m["M"],
-            m["name"]))
+            m["NS"], kind.get(m["name"], -1), m["name"]))
         n += 1
     for cls in wg:
         la, lb, lc, ld = cls
         for sph, pr in itertools.product(sph_modes, precs):
-            lines.append("    {%d, %d, %d, %d, %d, %d, 64, %d, %d, %d, 1, intc_eri_wg_%s_launch}," % (
+            lines.append("    {%d, %d, %d, %d, %d, %d, 64, %d, %d, %d, 1, 0, -1, intc_eri_wg_%s_launch}," % (
                 la, lb, lc, ld, int(sph), 0 if pr == "d" else 1, _nout(la, sph) * _nout(lb, sph),
                 _nout(lc, sph) * _nout(ld, sph), la + lb + lc + ld, pr))
             n += 1
@@ -732,4 +841,12 @@ def generate(outdir, classes, sph_modes=(False, True), precs=("d", "f"), wg=(), 
     if old != src:
         with open(tab, "w") as fh:
             fh.write(src)
+    if members:
+        src = emit_multi_source(members)
+        fn = os.path.join(outdir, "intc_eri_multi.hip")
+        old = open(fn).read() if os.path.exists(fn) else None
+        if old != src:
+            with open(fn, "w") as fh:
+                fh.write(src)
+        written.append(fn)
     return written, metas

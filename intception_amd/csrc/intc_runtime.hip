@@ -52,6 +52,10 @@ std::vector<DevTables> g_tables;  // per device
 
 // workgroup kernels (intc_eri_wg.hip): allow their dynamic LDS on this device
 int intc_wg_prepare();
+// persistent multi-class kernel (gen/intc_eri_multi.hip)
+extern "C" int intc_multi_prepare();
+extern "C" hipError_t intc_multi_launch(const IntcEriArgs *args, const int4 *tiles, int n_tiles, int *next, int reg,
+                                        int grid, hipStream_t st);
 
 namespace {
 
@@ -70,6 +74,7 @@ int ensure_tables(int dev, DevTables **out)
         HIPCHK(hipMalloc(&t.sph, sizeof(intc_builtin_sph)));
         HIPCHK(hipMemcpy(t.sph, intc_builtin_sph, sizeof(intc_builtin_sph), hipMemcpyHostToDevice));
         if (intc_wg_prepare()) return fail(-3, "workgroup kernels: cannot enable their LDS size");
+        if (intc_multi_prepare()) return fail(-3, "persistent kernel: cannot enable its LDS size");
     }
     *out = &t;
     return 0;
@@ -194,6 +199,21 @@ struct intc_plan {
     double *pairA = nullptr, *pairAB = nullptr;
     bool pairs_built = false;   // primitive-pair arrays written at least once
     bool route_fp32 = true;     // routed runs launch the FP32-recurrence queues
+    // persistent multi-class launch (intc_plan_set_persistent): the bins whose
+    // class the multi kernel holds run as tiles of one launch; one buffer set
+    // for eager execution and one bound by the captured graph
+    bool persistent = false;
+    int tile_iters = 64;                     // iterations of each team per tile (at most)
+    int multi_grid = 2 * 256;                // blocks of 256 (two per CU)
+    std::vector<long long> prim_prefix;      // primitive quartets before each plan position
+    struct MultiBufs {
+        IntcEriArgs *args = nullptr;         // [2 * bins]: bin b, precision r at 2b + r
+        int4 *tiles = nullptr;
+        int *next = nullptr;
+        int n_tiles = 0, cap_tiles = 0, reg = 0;
+        std::vector<IntcEriArgs> h_args;
+        std::vector<int4> h_tiles;
+    } m_eager, m_graph;
     // captured execution
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
@@ -220,6 +240,12 @@ void plan_free(intc_plan *p)
                     p->pairA, p->pairAB};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
+    for (intc_plan::MultiBufs *m : {&p->m_eager, &p->m_graph}) {
+        void *mp[] = {m->args, m->tiles, m->next};
+        for (void *q : mp)
+            if (q) (void)hipFree(q);
+        *m = intc_plan::MultiBufs();
+    }
 }
 
 // lanes of a wave doing useful work when teams of G*T lanes tile it
@@ -466,6 +492,8 @@ static int finish_plan(intc_plan *p, const std::vector<const IntcClassEntry *> &
     std::vector<int> pp_pair(npp);
     for (size_t pr = 0; pr < npair; ++pr)
         for (int k = 0; k < pair_np[pr]; ++k) pp_pair[pair_off[pr] + k] = (int)pr;
+    p->prim_prefix.assign(nq + 1, 0);
+    for (long long i = 0; i < nq; ++i) p->prim_prefix[i + 1] = p->prim_prefix[i] + prims[i];
     std::vector<int> bin_q0, q_bin(nq);
     for (long long i = 0; i < nq;) {
         long long j = i;
@@ -917,8 +945,150 @@ static int zero_skipped(intc_plan *p, double *out, hipStream_t st)
     return 0;
 }
 
+// Kernel arguments of one bin in one precision (the per-class launch and the
+// persistent kernel's argument array use the same fields).
+static IntcEriArgs bin_args(const intc_plan *p, const IntcBin &b, int prec, const IntcEriArgs &base, bool routed,
+                            int bi)
+{
+    IntcEriArgs a = base;
+    a.q_bra = p->q_bra + b.q0;
+    a.q_ket = p->q_ket + b.q0;
+    a.q_off = p->q_off + b.q0;
+    a.q_str = p->q_str + b.q0;
+    a.q_idx = routed ? (prec == 0 ? p->idx64 : p->idx32) + b.q0 : nullptr;
+    a.nq_dev = routed ? p->counts + 3 * bi + prec : nullptr;
+    a.nq = b.nq;
+    a.team_groups = b.team_groups[prec];
+    const IntcClassEntry *c = b.cls[prec];
+    a.wg_cls[0] = c->la;
+    a.wg_cls[1] = c->lb;
+    a.wg_cls[2] = c->lc;
+    a.wg_cls[3] = c->ld;
+    a.wg_cls[4] = c->sph;
+    return a;
+}
+
+// LDS doubles per wave of a register class kernel at team size T
+static int reg_doubles(const IntcClassEntry *c, int T)
+{
+    const int tpw = 64 / (c->G * T);
+    return std::max(tpw * c->NE * c->NCD, 64 * c->NS);
+}
+
+static bool multi_covers(const intc_plan *p, const IntcBin &b, int prec)
+{
+    return p->persistent && b.cls[prec]->multi >= 0;
+}
+
+// Argument array and tile list of the persistent launch.  A tile is a run of
+// one covered bin's queue positions, whole team iterations of (teams per
+// wave) quartets, at most tile_iters of them, closed early once its estimated
+// cost reaches 1/8 of an average wave's share of the launch (so that no tile
+// holds several of the heaviest quartets, which the per-class grids spread
+// over separate waves).  Routed tiles cover the whole bin for each queue
+// (the queue lengths are known on the device only; positions past a queue's
+// length end at once).  Device buffers are rewritten only when their contents
+// change, after the device has finished with them.
+static int prepare_multi(intc_plan *p, const double *sph, double *out, bool routed, intc_plan::MultiBufs &mb)
+{
+    if (sph) return fail(-1, "the persistent launch uses the built-in transforms: pass sph = NULL");
+    if (g_boys_mode != 0) return fail(-1, "the persistent launch uses the Chebyshev Boys function (mode 0)");
+    IntcEriArgs base;
+    int rc = fill_args(p, sph, out, &base);
+    if (rc) return rc;
+    const int nb = (int)p->bins.size();
+    std::vector<IntcEriArgs> args(2 * std::max(1, nb));
+    memset(args.data(), 0, sizeof(IntcEriArgs) * args.size());
+    const int nprec = routed ? (p->route_fp32 ? 2 : 1) : 1;
+    // cost model per quartet: per primitive quartet ~ G lanes x ket VRR size,
+    // plus the contracted epilogue ~ NE x NCD
+    auto per_prim = [](const IntcClassEntry *c) { return (double)c->G * ncc0(c->lc + c->ld) * (c->M + 1); };
+    auto per_q = [](const IntcClassEntry *c) { return 4.0 * c->NE * c->NCD; };
+    auto range_cost = [&](const IntcBin &b, const IntcClassEntry *c, long long lo, long long hi) {
+        return (double)(p->prim_prefix[b.q0 + hi] - p->prim_prefix[b.q0 + lo]) * per_prim(c) +
+               (double)(hi - lo) * per_q(c);
+    };
+    double total = 0.0;
+    for (int bi = 0; bi < nb; ++bi)
+        for (int prec = 0; prec < nprec; ++prec)
+            if (multi_covers(p, p->bins[bi], prec)) total += range_cost(p->bins[bi], p->bins[bi].cls[prec], 0,
+                                                                        p->bins[bi].nq);
+    const double cap = total / (8.0 * 4.0 * p->multi_grid);
+    std::vector<std::pair<double, int4>> tl;
+    int reg = 0;
+    for (int bi = 0; bi < nb; ++bi) {
+        const IntcBin &b = p->bins[bi];
+        for (int prec = 0; prec < nprec; ++prec) {
+            if (!multi_covers(p, b, prec)) continue;
+            const IntcClassEntry *c = b.cls[prec];
+            args[2 * bi + prec] = bin_args(p, b, prec, base, routed, bi);
+            const int T = b.team_groups[prec], tpw = 64 / (c->G * T);
+            reg = std::max(reg, reg_doubles(c, T));
+            for (long long lo = 0; lo < b.nq;) {
+                long long hi = lo;
+                double cost = 0.0;
+                for (int it = 0; it < p->tile_iters && hi < b.nq && (it == 0 || cost < cap); ++it) {
+                    const long long h2 = std::min<long long>(b.nq, hi + tpw);
+                    cost += range_cost(b, c, hi, h2);
+                    hi = h2;
+                }
+                tl.push_back({cost, make_int4(2 * bi + prec, c->multi, (int)lo, (int)hi)});
+                lo = hi;
+            }
+        }
+    }
+    // Tile order: bins interleaved (the k-th tile of every bin before any
+    // bin's (k+1)-th), so that the waves work on a mix of classes at any
+    // time; measured on MI355X against largest-cost-first and a random
+    // order (config 3: 49.0 vs 50.6 and 54.0 ms; config 1: 21.7 vs 27.7 and
+    // 23.7 ms, 64 iterations per tile).
+    {
+        std::vector<std::pair<long long, size_t>> key(tl.size());
+        std::vector<long long> cnt(args.size(), 0);
+        for (size_t i = 0; i < tl.size(); ++i) key[i] = {cnt[tl[i].second.x]++, i};
+        std::stable_sort(key.begin(), key.end());
+        std::vector<std::pair<double, int4>> t2(tl.size());
+        for (size_t i = 0; i < tl.size(); ++i) t2[i] = tl[key[i].second];
+        tl.swap(t2);
+    }
+    std::vector<int4> tiles(tl.size());
+    for (size_t i = 0; i < tl.size(); ++i) tiles[i] = tl[i].second;
+    if (reg > 160 * 1024 / 8 / 4) return fail(-1, "persistent launch: LDS per block exceeds 160 KB");
+    auto same4 = [](const std::vector<int4> &x, const std::vector<int4> &y) {
+        if (x.size() != y.size()) return false;
+        for (size_t i = 0; i < x.size(); ++i)
+            if (x[i].x != y[i].x || x[i].y != y[i].y || x[i].z != y[i].z || x[i].w != y[i].w) return false;
+        return true;
+    };
+    const bool args_same = mb.h_args.size() == args.size() &&
+                           memcmp(mb.h_args.data(), args.data(), sizeof(IntcEriArgs) * args.size()) == 0;
+    const bool tiles_same = same4(mb.h_tiles, tiles);
+    if (args_same && tiles_same && mb.next) return 0;
+    // the previous launches that read these buffers must have finished
+    HIPCHK(hipDeviceSynchronize());
+    if (!mb.args || mb.h_args.size() != args.size()) {
+        if (mb.args) HIPCHK(hipFree(mb.args));
+        mb.args = nullptr;
+        HIPCHK(hipMalloc(&mb.args, sizeof(IntcEriArgs) * args.size()));
+    }
+    if (!mb.tiles || mb.cap_tiles < (int)std::max<size_t>(1, tiles.size())) {
+        if (mb.tiles) HIPCHK(hipFree(mb.tiles));
+        mb.tiles = nullptr;
+        mb.cap_tiles = (int)std::max<size_t>(1, tiles.size());
+        HIPCHK(hipMalloc(&mb.tiles, sizeof(int4) * mb.cap_tiles));
+    }
+    if (!mb.next) HIPCHK(hipMalloc(&mb.next, sizeof(int)));
+    HIPCHK(hipMemcpy(mb.args, args.data(), sizeof(IntcEriArgs) * args.size(), hipMemcpyHostToDevice));
+    if (!tiles.empty()) HIPCHK(hipMemcpy(mb.tiles, tiles.data(), sizeof(int4) * tiles.size(), hipMemcpyHostToDevice));
+    mb.h_args = args;
+    mb.h_tiles = tiles;
+    mb.n_tiles = (int)tiles.size();
+    mb.reg = std::max(reg, 1);
+    return 0;
+}
+
 static int run_all(intc_plan *p, const double *sph, double *out, const signed char *route, int n_streams,
-                   hipStream_t st)
+                   hipStream_t st, const intc_plan::MultiBufs *mb = nullptr)
 {
     IntcEriArgs a;
     int rc = fill_args(p, sph, out, &a);
@@ -947,10 +1117,19 @@ static int run_all(intc_plan *p, const double *sph, double *out, const signed ch
                (double)p->bins[y].nq * p->bins[y].cls[0]->NE * p->bins[y].cls[0]->NCD;
     });
     int k = 0;
+    if (mb) {
+        // the persistent kernel on the launch stream; uncovered bins on the others
+        if (mb->n_tiles > 0) {
+            hipError_t e = intc_multi_launch(mb->args, mb->tiles, mb->n_tiles, mb->next, mb->reg, p->multi_grid, st);
+            if (e != hipSuccess) return fail(-3, std::string("persistent kernel launch: ") + hipGetErrorString(e));
+        }
+        k = n_streams > 1 ? 1 : 0;
+    }
     const int nprec = route ? (p->route_fp32 ? 2 : 1) : 1;
     for (int bi : ord) {
         const IntcBin &b = p->bins[bi];
         for (int prec = 0; prec < nprec; ++prec) {
+            if (mb && multi_covers(p, b, prec)) continue;
             hipStream_t s2 = (k % n_streams) == 0 ? st : p->side[(k % n_streams) - 1];
             ++k;
             const int *qi = route ? (prec == 0 ? p->idx64 : p->idx32) + b.q0 : nullptr;
@@ -993,9 +1172,35 @@ static int ensure_streams(intc_plan *p, int n_streams)
 INTC_API int intc_plan_execute(intc_plan *p, const double *exps, const double *coefs, const double *shell_cen,
                                const double *sph, double *out, void *stream)
 {
+    if (p->persistent) {
+        int rc = prepare_multi(p, sph, out, false, p->m_eager);
+        if (rc) return rc;
+    }
     int rc = intc_plan_build_pairs(p, exps, coefs, shell_cen, stream);
     if (rc) return rc;
-    return run_all(p, sph, out, nullptr, 1, (hipStream_t)stream);
+    return run_all(p, sph, out, nullptr, 1, (hipStream_t)stream, p->persistent ? &p->m_eager : nullptr);
+}
+
+// Persistent multi-class launch for execute, execute_routed and graphs
+// captured afterwards: the bins whose class the multi kernel holds run as
+// tiles of one launch (spherical plans, built-in transforms, Chebyshev Boys).
+INTC_API int intc_plan_set_persistent(intc_plan *p, int enable, int tile_iters, int grid)
+{
+    if (enable && !p->spherical) return fail(-2, "the persistent kernel holds the spherical class kernels only");
+    if (tile_iters < 1 || tile_iters > (1 << 20)) return fail(-1, "tile_iters must be in [1, 2^20]");
+    if (grid < 0 || grid > 256 * 64) return fail(-1, "grid must be in [0, 16384] (0: two blocks per CU)");
+    p->persistent = enable != 0;
+    p->tile_iters = tile_iters;
+    p->multi_grid = grid ? grid : 2 * 256;
+    return 0;
+}
+
+// Tiles and LDS doubles per wave of the last persistent eager execution.
+INTC_API int intc_plan_persistent_info(const intc_plan *p, int *n_tiles, int *reg_doubles)
+{
+    *n_tiles = p->m_eager.n_tiles;
+    *reg_doubles = p->m_eager.reg;
+    return 0;
 }
 
 INTC_API int intc_plan_execute_routed(intc_plan *p, const double *exps, const double *coefs,
@@ -1003,9 +1208,13 @@ INTC_API int intc_plan_execute_routed(intc_plan *p, const double *exps, const do
                                       double *out, void *stream)
 {
     if (!route) return fail(-1, "route array is required");
+    if (p->persistent) {
+        int rc = prepare_multi(p, sph, out, true, p->m_eager);
+        if (rc) return rc;
+    }
     int rc = intc_plan_build_pairs(p, exps, coefs, shell_cen, stream);
     if (rc) return rc;
-    return run_all(p, sph, out, route, 1, (hipStream_t)stream);
+    return run_all(p, sph, out, route, 1, (hipStream_t)stream, p->persistent ? &p->m_eager : nullptr);
 }
 
 // Capture pair building, the optional route compaction and all class bins
@@ -1027,20 +1236,25 @@ INTC_API int intc_plan_graph_create(intc_plan *p, const double *exps, const doub
     }
     int rc = ensure_streams(p, n_streams);
     if (rc) return rc;
+    {
+        DevTables *t;
+        rc = ensure_tables(p->device, &t);
+        if (rc) return rc;
+    }
+    if (p->persistent) {
+        // the graph binds its own argument and tile buffers
+        rc = prepare_multi(p, sph, out, route != nullptr, p->m_graph);
+        if (rc) return rc;
+    }
     hipStream_t st = (hipStream_t)stream;
     bool own = false;
     if (!st) {
         HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
         own = true;
     }
-    {
-        DevTables *t;
-        rc = ensure_tables(p->device, &t);
-        if (rc) return rc;
-    }
     HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
     rc = intc_plan_build_pairs(p, exps, coefs, shell_cen, st);
-    if (!rc) rc = run_all(p, sph, out, route, n_streams, st);
+    if (!rc) rc = run_all(p, sph, out, route, n_streams, st, p->persistent ? &p->m_graph : nullptr);
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(st, &g);
     if (own) (void)hipStreamDestroy(st);

@@ -138,6 +138,21 @@ class EriPlan:
     def set_route_fp32(self, use_fp32):
         _lib.check(self._L.intc_plan_set_route_fp32(self.handle, int(bool(use_fp32))))
 
+    def set_persistent(self, enable=True, tile_iters=64, grid=0):
+        """One persistent launch for every bin whose class the multi-class
+        kernel holds (spherical, built-in transforms, Chebyshev Boys): waves
+        take tiles of (teams per wave) x (at most tile_iters) quartet
+        positions from a device counter, bins interleaved.  grid = 0: two
+        blocks of 256 per CU.  Applies to run, run_routed and later
+        captures."""
+        _lib.check(self._L.intc_plan_set_persistent(self.handle, int(bool(enable)), int(tile_iters), int(grid)))
+
+    def persistent_info(self):
+        """(tiles, LDS doubles per wave) of the last persistent eager run."""
+        n, r = ctypes.c_int(), ctypes.c_int()
+        _lib.check(self._L.intc_plan_persistent_info(self.handle, ctypes.byref(n), ctypes.byref(r)))
+        return n.value, r.value
+
     def route_counts(self):
         n = self._L.intc_plan_num_bins(self.handle)
         buf = (ctypes.c_int * (3 * max(n, 1)))()
