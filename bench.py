@@ -92,6 +92,24 @@ def measured_traffic(kernel, workload):
     return float(t["bytes_per_launch"]) if t and t.get("workload") == workload else None
 
 
+def measured_valu_flops(kernel, workload, kernel_ms):
+    """Hardware FP64 VALU FLOPs per launch of `kernel` from the committed
+    counter run (profiles/traffic.json: 64 x SQ_INSTS_VALU_FLOPS_FP64), and
+    the rate and fraction of the FP64 peak they give at this run's launch
+    time; None when that kernel/workload was not measured."""
+    try:
+        with open(os.path.join(ROOT, "profiles", "traffic.json")) as fh:
+            t = json.load(fh).get(kernel)
+    except (OSError, ValueError):
+        return None
+    if not t or t.get("workload") != workload or "valu_flops_per_launch" not in t:
+        return None
+    f = float(t["valu_flops_per_launch"])
+    tf = f / (kernel_ms * 1e-3) / 1e12
+    return {"flops_per_launch": f, "achieved": tf, "frac": tf / FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "source": t["valu_flops_source"]}
+
+
 def _timed(torch, dev, stream, fn, n):
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize(dev)
@@ -291,6 +309,8 @@ def run_molecule(args, three_center=False):
                      "kernel_flops": dom["flops"],
                      "note": "dominant FP64 class kernel of the plain path; FP64 peak 78.6 TF/s (vector = "
                              "matrix on MI355X); flops = reference-scheme count (intception_amd/flops_model.py)",
+                     "rocprof_valu_fp64": measured_valu_flops(dom_name, "config4" if three_center else "config3",
+                                                              dom["ms"]),
                      "job_frac_plain_fp64": float(flops_q.sum() / (ms64 * 1e-3) / 1e12 / FP64_PEAK_TFLOPS)},
         "cpu_baseline": cpu,
         "setup_s": {"router_train_and_calibrate": t_router, "molecule_and_screening": t_setup, "plan": t_plan},
@@ -472,9 +492,10 @@ def run_config1(args):
 
 def traffic_probe(args):
     """Counter-run helper: builds the config-3 plan, runs the plain path once,
-    then launches the class kernel of args.probe_class once more and one
-    elementwise kernel that reads 1 GiB and writes 1 GiB (the scale of the
-    FETCH_SIZE / WRITE_SIZE counters).  Prints one JSON line."""
+    then launches the FP64 and the FP32-recurrence kernel of the class
+    args.probe_class once each, and one elementwise kernel that reads 1 GiB,
+    writes 1 GiB and does 2^27 FP64 additions (the scale of the FETCH_SIZE /
+    WRITE_SIZE and FLOP counters).  Prints one JSON line."""
     import torch
     from intception_amd import autotune, lib, runtime, screening
     lib.check(lib.load().intc_init())
@@ -488,16 +509,18 @@ def traffic_probe(args):
     torch.cuda.synchronize()
     cls = tuple(int(c) for c in args.probe_class)
     hit = [r for r in plan.bins(0) if tuple(r["cls"]) == cls]
-    for r in hit:
-        plan.run_bin(r["bin"], out, 0)
+    for prec in (0, 1):
+        for r in hit:
+            plan.run_bin(r["bin"], out, prec)
     x = torch.ones(1 << 27, dtype=torch.float64, device=plan.device)
     y = torch.empty_like(x)
     torch.add(x, 1.0, out=y)
     torch.cuda.synchronize()
     nq = sum(r["nq"] for r in hit)
     plan.close()
-    return {"probe": "class kernel intc_eri_d%d%d%d%d_s_bs once (%d quartets) + torch.add over 2^27 doubles "
-                     "(1 GiB read, 1 GiB written)" % (cls + (nq,)), "quartets": nq}
+    return {"probe": "class kernels intc_eri_d%d%d%d%d_s_bs (FP64) then intc_eri_f%d%d%d%d_s_bs (FP32 "
+                     "recurrences) once each (%d quartets) + torch.add over 2^27 doubles (1 GiB read, 1 GiB "
+                     "written, 2^27 FP64 additions)" % (cls + cls + (nq,)), "quartets": nq}
 
 
 def main():
