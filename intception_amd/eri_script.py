@@ -15,7 +15,13 @@ from .dsl import (boys_f, dsl_cartesian_gaussian, dsl_integer_index, dsl_integra
                   dsl_value, exp, sqrt)
 
 
-def make_eri(vrr_indices=("a", "c", "d")):
+def make_eri(vrr_indices=("a", "c", "d"), hrr=True):
+    """vrr_indices: the indices built by Obara-Saika VRRs; hrr: add the HGP HRR
+    moving angular momentum from a to b.  The default is the VRR+HRR scheme
+    (src/intception/generator/callables/main_source_algorithms/
+    contracted_vrr_hrr_with_auxiliary.py); VRRs on a, b, c, d without the HRR
+    is the reference's VRR-only variant (contracted_vrr_only_with_auxiliary.py),
+    the same integrals by another recurrence path."""
     ga = dsl_cartesian_gaussian("a", constant=True)
     gb = dsl_cartesian_gaussian("b", constant=True)
     gc = dsl_cartesian_gaussian("c", constant=True)
@@ -81,7 +87,8 @@ def make_eri(vrr_indices=("a", "c", "d")):
     for k in vrr_indices:
         idx, expr = vrr[k]
         eri.add_vrr("vrr_" + k, idx, expr)
-    eri.add_hrr("hrr", ga, gb)
+    if hrr:
+        eri.add_hrr("hrr", ga, gb)
     return eri
 
 

@@ -23,6 +23,20 @@ def test_eri_script_recognised():
     assert eri.full_name() == "intception_integral_eri_abcd_m"
 
 
+def test_vrr_only_variant_recognised():
+    """VRRs on a, b, c and d with no HRR (the reference's
+    contracted_vrr_only_with_auxiliary algorithm): the same integral family,
+    bound to the same kernels; a VRR-only script missing one index is
+    rejected."""
+    eri = eri_script.make_eri(("a", "b", "c", "d"), hrr=False)
+    assert recognize_eri(eri)
+    assert [r.rrtype() for r in eri.rr_list()] == ["vrr"] * 4
+    gen = generator(eri, opt=generator_options(contracted=True, spherical_transformed=True, backend="hip"))
+    assert gen.names() == ["intception_integral_eri_abcd_m_cont_sph"]
+    with pytest.raises(NotImplementedError):
+        recognize_eri(eri_script.make_eri(("a", "c", "d"), hrr=False))
+
+
 def test_broken_scripts_rejected():
     import intception_amd.dsl as d
     eri = eri_script.make_eri()
