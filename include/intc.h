@@ -147,6 +147,53 @@ int intc_plan_route_counts(intc_plan *plan, int *counts);
 int intc_plan_quartet_order(intc_plan *plan, int *order);
 int intc_plan_destroy(intc_plan *plan);
 
+/* ---------------------------------------------------------------------- */
+/* One-electron kinds of the input-script API: the 2-index overlap (a|b)  */
+/* and nuclear attraction (a|C|b) of the reference's example script       */
+/* (examples/example_input.py:79-110), shells up to l = 3.               */
+/* ---------------------------------------------------------------------- */
+/* Replace the <full>_main the reference generates for that script with
+ * generator_options(contracted=False) (primitive, Cartesian): per index the
+ * exponent, centre, l, n, iskip (primitive_general.py:31-98, attachment order
+ * dsl.py:971-972); the nuclear attraction adds the centre C as the integral's
+ * extra argument after the indices.  Output out[ia*iskipa + ib*iskipb]. */
+int intc_overlap_prim_cart_main(const double xa, const double ca[3], const int la, const int na, const int iskipa,
+                                const double xb, const double cb[3], const int lb, const int nb, const int iskipb,
+                                double *work_array, double *output_array);
+int intc_nuclear_prim_cart_main(const double xa, const double ca[3], const int la, const int na, const int iskipa,
+                                const double xb, const double cb[3], const int lb, const int nb, const int iskipb,
+                                const double cc[3], double *work_array, double *output_array);
+/* Contracted variants (contracted=True): the per-index argument layout of the
+ * ERI twins above (contracted_general.py:31-73) for two indices. */
+int intc_overlap_cont_cart_main(INTC_SHELL_ARGS(a), INTC_SHELL_ARGS(b), double *work_array, double *output_array);
+int intc_overlap_cont_sph_main(INTC_SHELL_ARGS(a), const double *sph_trans_array_a, INTC_SHELL_ARGS(b),
+                               const double *sph_trans_array_b, double *work_array, double *output_array);
+int intc_nuclear_cont_cart_main(INTC_SHELL_ARGS(a), INTC_SHELL_ARGS(b), const double cc[3], double *work_array,
+                                double *output_array);
+int intc_nuclear_cont_sph_main(INTC_SHELL_ARGS(a), const double *sph_trans_array_a, INTC_SHELL_ARGS(b),
+                               const double *sph_trans_array_b, const double cc[3], double *work_array,
+                               double *output_array);
+
+/* Batched twin (no counterpart in the reference): kind 0 = overlap, items
+ * are n_item shell pairs (a, b); kind 1 = nuclear attraction, items are
+ * n_item triples (a, b, c), c indexing n_centre point-charge centres.  Item i
+ * writes a row-major [a][b] block at out[out_off[i]] (Cartesian or 2l+1
+ * spherical components).  Host index arrays; device memory owned by the plan. */
+typedef struct intc_onee_plan intc_onee_plan;
+/* shell_cen, exps, coefs: host copies (may be NULL); when given, each pair is
+ * built on the shell its primitive-pair centres lie closest to and the HRR
+ * moves angular momentum to the other (as for the ERI plans). */
+int intc_onee_plan_create(intc_onee_plan **plan, int kind, int spherical, int n_shell, const int *shell_l,
+                          const int *shell_nprim, const int *shell_off, const double *shell_cen, const double *exps,
+                          const double *coefs, int n_centre, long long n_item, const int *items,
+                          const long long *out_off);
+/* Device pointers: exps, coefs (normalised, one contraction per shell),
+ * shell_cen [3 n_shell], centres [3 n_centre] (NULL for overlap), sph (S_l
+ * table as above; NULL = built-in real solid harmonics), out. */
+int intc_onee_plan_execute(intc_onee_plan *plan, const double *exps, const double *coefs, const double *shell_cen,
+                           const double *centres, const double *sph, double *out, void *stream);
+int intc_onee_plan_destroy(intc_onee_plan *plan);
+
 #ifdef __cplusplus
 }
 #endif

@@ -696,6 +696,171 @@ def _nout(l, sph):
     return 2 * l + 1 if sph else (l + 1) * (l + 2) // 2
 
 
+# ---------------------------------------------------------------------------
+# one-electron integrals: overlap (a|b) and nuclear attraction (a|C|b)
+# ---------------------------------------------------------------------------
+ONEE_LMAX = 3
+ONEE_KINDS = ("o", "v")   # overlap, nuclear attraction (point charge at C)
+
+
+def onee_kernel_name(kind, la, lb, spherical):
+    return "intc_onee_%s%d%d_%s" % (kind, la, lb, "s" if spherical else "c")
+
+
+def emit_onee_kernel(o, kind, la, lb, spherical):
+    """One thread per contracted item.  Obara-Saika VRR on a, (e|0)^(m) for
+    |e| <= la + lb in registers, with the reference's direction rule (first
+    non-zero component); contraction into la <= |e| accumulators; HGP HRR a -> b
+    on the contracted values; optional Cartesian -> spherical step with the
+    S_l of a.sph.  Overlap: base (pi/zeta)^(3/2) exp(-mu R^2); nuclear
+    attraction: base 2 pi/zeta exp(-mu R^2) F_m(zeta |P-C|^2) and the
+    -(P-C)_i (e-1_i)^(m+1) terms (the recurrences of the reference's example
+    script, examples/example_input.py:79-110, with the VRR on a and the
+    transfer to b by the HRR)."""
+    L = la + lb
+    nuc = kind == "v"
+    M = L if nuc else 0
+    name = onee_kernel_name(kind, la, lb, spherical)
+    o("template <int BM>")
+    o("__device__ __forceinline__ void %s_body(const IntcOneArgs &a)" % name)
+    o("{")
+    o.ind = 1
+    o("constexpr int M = %d;" % M)
+    o("(void)M;")
+    o("for (int it = blockIdx.x * blockDim.x + threadIdx.x; it < a.n; it += gridDim.x * blockDim.x) {")
+    o.ind = 2
+    o("const int sa = a.it_a[it], sb = a.it_b[it];")
+    o("const double Ax = a.shell_cen[3 * sa], Ay = a.shell_cen[3 * sa + 1], Az = a.shell_cen[3 * sa + 2];")
+    o("const double Bx = a.shell_cen[3 * sb], By = a.shell_cen[3 * sb + 1], Bz = a.shell_cen[3 * sb + 2];")
+    if nuc:
+        o("const int ic = a.it_c[it];")
+        o("const double Cx = a.centres[3 * ic], Cy = a.centres[3 * ic + 1], Cz = a.centres[3 * ic + 2];")
+    o("const int oa = a.shell_off[sa], na = a.shell_np[sa], ob = a.shell_off[sb], nb = a.shell_np[sb];")
+    o("const double ABx = Ax - Bx, ABy = Ay - By, ABz = Az - Bz;")
+    o("const double R2 = ABx * ABx + ABy * ABy + ABz * ABz;")
+    o("(void)ABx; (void)ABy; (void)ABz;")
+    acc = [e for e in comps_upto(L) if sum(e) >= la]
+    for e in acc:
+        o("double acc%d = 0.0;" % cidx(e))
+    o("for (int i = 0; i < na; ++i) {")
+    o.ind = 3
+    o("const double al = a.exps[oa + i], ca = a.coefs[oa + i];")
+    o("for (int j = 0; j < nb; ++j) {")
+    o.ind = 4
+    o("const double be = a.exps[ob + j], cb = a.coefs[ob + j];")
+    o("const double z = al + be, oz = 1.0 / z, o2z = 0.5 * oz;")
+    o("(void)o2z;")
+    o("const double Px = (al * Ax + be * Bx) * oz, Py = (al * Ay + be * By) * oz, Pz = (al * Az + be * Bz) * oz;")
+    o("const double PAx = Px - Ax, PAy = Py - Ay, PAz = Pz - Az;")
+    o("(void)PAx; (void)PAy; (void)PAz;")
+    o("const double kab = ca * cb * exp(-al * be * oz * R2);")
+    if nuc:
+        o("const double PCx = Px - Cx, PCy = Py - Cy, PCz = Pz - Cz;")
+        o("(void)PCx; (void)PCy; (void)PCz;")
+        o("const double Tb = z * (PCx * PCx + PCy * PCy + PCz * PCz);")
+        o("double F[M + 1];")
+        o("intc_boys<M, BM>(Tb, F, a);")
+        o("const double pref = kab * (2.0 * 3.141592653589793238) * oz;")
+        for m in range(M + 1):
+            o("const double v0_%d = pref * F[%d];" % (m, m))
+    else:
+        # (pi / zeta)^(3/2)
+        o("const double v0_0 = kab * 5.568327996831707845 * oz * sqrt(oz);")
+    for n in range(1, L + 1):
+        for e in cart_components(n):
+            i = first_nonzero(e)
+            e1 = dec(e, i)
+            k = e1[i]
+            ax = "xyz"[i]
+            for m in (range(M - n + 1) if nuc else range(1)):
+                if nuc:
+                    expr = "PA%s * v%d_%d - PC%s * v%d_%d" % (ax, cidx(e1), m, ax, cidx(e1), m + 1)
+                else:
+                    expr = "PA%s * v%d_%d" % (ax, cidx(e1), m)
+                if k > 0:
+                    e2 = dec(e1, i)
+                    if nuc:
+                        expr += " + %d.0 * o2z * (v%d_%d - v%d_%d)" % (k, cidx(e2), m, cidx(e2), m + 1)
+                    else:
+                        expr += " + %d.0 * o2z * v%d_%d" % (k, cidx(e2), m)
+                o("const double v%d_%d = %s;" % (cidx(e), m, expr))
+    for e in acc:
+        o("acc%d += v%d_0;" % (cidx(e), cidx(e)))
+    o.ind = 3
+    o("}")
+    o.ind = 2
+    o("}")
+    hl, H = _hrr_lines(lambda e: "acc%d" % cidx(e), la, lb, "AB", "h", "bra")
+    for ln in hl:
+        o(ln)
+    o("double *op = a.out + a.it_off[it];")
+    o("const unsigned long long st = a.it_str[it];")
+    o("const long long s_a = (long long)(st & 0xffffffffull), s_b = (long long)(st >> 32);")
+    ac, bc = cart_components(la), cart_components(lb)
+    if spherical:
+        o("const double *SA = a.sph + %d, *SB = a.sph + %d;" % (intc_sph_off(la), intc_sph_off(lb)))
+        for ai, aa in enumerate(ac):
+            for sbb in range(nsph(lb)):
+                terms = " + ".join("%s * SB[%d]" % (H(aa, bb), bi + sbb * ncc(lb)) for bi, bb in enumerate(bc))
+                o("const double m1_%d_%d = %s;" % (ai, sbb, terms))
+        for saa in range(nsph(la)):
+            for sbb in range(nsph(lb)):
+                terms = " + ".join("SA[%d] * m1_%d_%d" % (ai + saa * ncc(la), ai, sbb) for ai in range(len(ac)))
+                o("op[%d * s_a + %d * s_b] = %s;" % (saa, sbb, terms))
+    else:
+        for ai, aa in enumerate(ac):
+            for bi, bb in enumerate(bc):
+                o("op[%d * s_a + %d * s_b] = %s;" % (ai, bi, H(aa, bb)))
+    o.ind = 1
+    o("}")
+    o.ind = 0
+    o("}")
+    o('extern "C" __global__ void __launch_bounds__(256) %s(const IntcOneArgs a) { %s_body<0>(a); }' % (name, name))
+    if nuc:
+        o('extern "C" __global__ void __launch_bounds__(256) %s_refboys(const IntcOneArgs a) { %s_body<1>(a); }'
+          % (name, name))
+    o("static hipError_t %s_launch(const IntcOneArgs *a, int grid, hipStream_t st)" % name)
+    o("{")
+    if nuc:
+        o("    if (a->boys_mode == 1)")
+        o("        hipLaunchKernelGGL(%s_refboys, dim3(grid), dim3(256), 0, st, *a);" % name)
+        o("    else")
+        o("        hipLaunchKernelGGL(%s, dim3(grid), dim3(256), 0, st, *a);" % name)
+    else:
+        o("    hipLaunchKernelGGL(%s, dim3(grid), dim3(256), 0, st, *a);" % name)
+    o("    return hipGetLastError();")
+    o("}")
+    o("")
+
+
+def emit_onee_source(lmax=ONEE_LMAX):
+    """All one-electron class kernels (both kinds, l <= lmax on each shell,
+    Cartesian and spherical) and their dispatch table, in one file."""
+    o = Emitter()
+    o.ind = 0
+    o("// Generated by intception_amd/codegen.py -- do not edit.")
+    o("// One-electron class kernels: overlap (o) and nuclear attraction (v).")
+    o('#include "../intc_common.h"')
+    o("")
+    entries = []
+    for kind in ONEE_KINDS:
+        for la in range(lmax + 1):
+            for lb in range(lmax + 1):
+                for sph in (False, True):
+                    emit_onee_kernel(o, kind, la, lb, sph)
+                    entries.append((kind, la, lb, sph))
+    o("typedef hipError_t (*intc_one_launch_fn)(const IntcOneArgs *, int, hipStream_t);")
+    o("struct IntcOneEntry { int kind, la, lb, sph; intc_one_launch_fn fn; };")
+    o("static const IntcOneEntry intc_one_table[] = {")
+    for kind, la, lb, sph in entries:
+        o("    {%d, %d, %d, %d, %s_launch}," % (ONEE_KINDS.index(kind), la, lb, int(sph),
+                                               onee_kernel_name(kind, la, lb, sph)))
+    o("};")
+    o("static const int intc_n_one = %d;" % len(entries))
+    o("static const int intc_one_lmax = %d;" % lmax)
+    return o.text() + "\n"
+
+
 def multi_members(metas):
     """Class kernels compiled into the persistent multi-class kernel: every
     spherical register kernel, FP64 and FP32 recurrences, except the FP32
@@ -840,6 +1005,12 @@ def generate(outdir, classes, sph_modes=(False, True), precs=("d", "f"), wg=(), 
     old = open(tab).read() if os.path.exists(tab) else None
     if old != src:
         with open(tab, "w") as fh:
+            fh.write(src)
+    src = emit_onee_source()
+    fn = os.path.join(outdir, "onee_kernels.inc")
+    old = open(fn).read() if os.path.exists(fn) else None
+    if old != src:
+        with open(fn, "w") as fh:
             fh.write(src)
     if members:
         src = emit_multi_source(members)

@@ -46,6 +46,25 @@ struct IntcEriArgs {
     int sph_builtin;                  // 1: sph is the built-in table (register kernels use literals)
 };
 
+// One-electron integrals (overlap, nuclear attraction) of contracted shell
+// pairs: item i is (shell it_a[i] | [centre it_c[i]] | shell it_b[i]) and
+// writes out[it_off[i] + ia * s_a + ib * s_b] with s_a = it_str[i] & 0xffffffff,
+// s_b = it_str[i] >> 32 over the components of a and b.
+struct IntcOneArgs {
+    const int *it_a, *it_b, *it_c;
+    const long long *it_off;
+    const unsigned long long *it_str;
+    int n;                            // items of this class bin
+    int boys_mode;
+    const int *shell_l, *shell_np, *shell_off;
+    const double *shell_cen;          // [n_shell * 3]
+    const double *exps, *coefs;       // concatenated primitive data (normalised coefficients)
+    const double *centres;            // [n_centre * 3] point-charge centres (nuclear attraction)
+    const double *sph;                // S_l, l = 0..7 (the IntcEriArgs::sph layout)
+    const double *boys_cheb, *boys_small, *boys_medium;
+    double *out;
+};
+
 // offset of S_l inside IntcEriArgs::sph: sum_{l' < l} ncc(l') (2l' + 1)
 __host__ __device__ constexpr int intc_sph_off(int l)
 {
@@ -73,8 +92,10 @@ __device__ __forceinline__ float intc_shflf(float v, int src4)
 // the exp(-T) term beyond.  Mode 1: the reference's three-branch scheme
 // (src/intception/generator/callables/generator_boys_function.py:98-205) on
 // grid tables of the same shape as the reference's.
-template <int M, int BM>
-__device__ __forceinline__ void intc_boys(double T, double *F, const IntcEriArgs &a)
+// Args: any argument struct with the boys_cheb / boys_small / boys_medium
+// table pointers (IntcEriArgs, IntcOneArgs).
+template <int M, int BM, class Args>
+__device__ __forceinline__ void intc_boys(double T, double *F, const Args &a)
 {
     // T = rho |PQ|^2 >= 0; the clamp (fmax maps NaN to 0) keeps every table
     // index below in range whatever the input

@@ -454,6 +454,26 @@ INTC_API int intc_set_boys_mode(int mode)
     return 0;
 }
 
+// shared with the one-electron plans (intc_onee.hip): error message and the
+// device tables of the current device
+int intc_internal_fail(int code, const char *msg) { return fail(code, msg); }
+
+int intc_internal_tables(const double **cheb, const double **small, const double **medium, const double **sph)
+{
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    DevTables *t;
+    int rc = ensure_tables(dev, &t);
+    if (rc) return rc;
+    *cheb = t->cheb;
+    *small = t->small;
+    *medium = t->medium;
+    *sph = t->sph;
+    return 0;
+}
+
+int intc_internal_boys_mode() { return g_boys_mode; }
+
 INTC_API int intc_init(void)
 {
     int dev = 0;

@@ -12,7 +12,9 @@ With ``backend="hip"`` the generator does not emit recurrence code from the
 script's expression tree.  It checks, numerically at random exponents,
 centres, target components and directions, that each integral is the
 Obara-Saika / Head-Gordon-Pople Coulomb ERI family (base expression, every VRR
-and the HRR), and binds it to the hand-written gfx950 kernels of libintc:
+and the HRR) -- or, for 2-index integrals, the Obara-Saika overlap or nuclear
+attraction of the reference's example script (``OneElectronBinding``) -- and
+binds it to the hand-written gfx950 kernels of libintc:
 
 * ``gen.binding(name)`` -> ``EriBinding`` with the per-quartet twin
   ``main(...)`` (same argument layout as the reference's ``<full>_main``),
@@ -229,6 +231,123 @@ def recognize_eri(integral, trials=24, seed=7):
 
 
 # ---------------------------------------------------------------------------
+# numeric recognition of the 2-index overlap and nuclear attraction families
+# (the reference's example script, examples/example_input.py:79-110)
+# ---------------------------------------------------------------------------
+def _expected_onee_base(kind, ex, cen, names, m, C):
+    a, b = (ex[n] for n in names)
+    A, B = (cen[n] for n in names)
+    z = a + b
+    P = (a * A + b * B) / z
+    k = math.exp(-a * b / z * float(np.sum((A - B) ** 2)))
+    if kind == "overlap":
+        return (math.pi / z) ** 1.5 * k
+    return 2 * math.pi / z * k * _boys_exact(m, z * float(np.sum((P - C) ** 2)))
+
+
+def _expected_onee_vrr(kind, ex, cen, names, aux, X, target, i, C):
+    """OS VRR of the overlap / nuclear attraction incrementing index X (0 or
+    1) to `target` in direction i: {shift key: coefficient}."""
+    a, b = (ex[n] for n in names)
+    A, B = (cen[n] for n in names)
+    z = a + b
+    P = (a * A + b * B) / z
+    partner = 1 - X
+    nuc = kind == "nuclear"
+    out = {}
+
+    def key(dec):
+        sh = tuple((n, -dec.get(k, 0)) for k, n in enumerate(names))
+        return sh + (((aux, dec.get("m", 0)),) if nuc else ())
+
+    def add(dec, v):
+        if v != 0.0:
+            k = key(dec)
+            out[k] = out.get(k, 0.0) + v
+
+    add({X: 1}, (P - [A, B][X])[i])
+    if nuc:
+        add({X: 1, "m": 1}, -(P - C)[i])
+    nx = target[names[X]][i] - 1
+    if nx > 0:
+        add({X: 2}, nx / (2 * z))
+        if nuc:
+            add({X: 2, "m": 1}, -nx / (2 * z))
+    npart = target[names[partner]][i]
+    if npart > 0:
+        add({X: 1, partner: 1}, npart / (2 * z))
+        if nuc:
+            add({X: 1, partner: 1, "m": 1}, -npart / (2 * z))
+    return out
+
+
+def recognize_onee(integral, trials=24, seed=11):
+    """'overlap' or 'nuclear' when `integral` is the Obara-Saika 2-index
+    overlap (two Cartesian indices) or nuclear attraction (two indices, one
+    auxiliary index, one position argument: the point charge); raises
+    NotImplementedError otherwise."""
+    idx, aux, args = integral.index_list(), integral.aux_list(), integral.arg_list()
+    if len(idx) != 2:
+        raise NotImplementedError("hip backend: %s is not a 2-index integral" % integral.name())
+    if not aux and not args:
+        kind = "overlap"
+    elif len(aux) == 1 and len(args) == 1 and isinstance(args[0], dsl.dsl_position):
+        kind = "nuclear"
+    else:
+        raise NotImplementedError("hip backend: 2-index integral %s is neither the overlap nor the nuclear "
+                                  "attraction" % integral.name())
+    names = [i.name() for i in idx]
+    mname = aux[0].name() if aux else None
+    cname = args[0].name() if args else None
+    rng = np.random.default_rng(seed)
+    base = integral.base()
+    if base is None:
+        raise NotImplementedError("no base expression")
+
+    def ctx_c():
+        ctx, ex, cen = _ctx(rng, names)
+        C = rng.uniform(-1.5, 1.5, size=3)
+        if cname:
+            ctx["vars"][cname] = C
+        return ctx, ex, cen, C
+
+    for _ in range(trials // 4):
+        ctx, ex, cen, C = ctx_c()
+        for m in range(4 if kind == "nuclear" else 1):
+            ctx["aux"] = {mname: m} if mname else {}
+            got = base.ev(ctx)
+            ref = _expected_onee_base(kind, ex, cen, names, m, C)
+            if not abs(got - ref) <= 1e-11 * abs(ref) + 1e-300:
+                raise NotImplementedError("base expression of %s is not the %s base" % (integral.name(), kind))
+    vrrs = [r for r in integral.rr_list() if r.rrtype() == "vrr"]
+    hrrs = [r for r in integral.rr_list() if r.rrtype() == "hrr"]
+    if not vrrs:
+        raise NotImplementedError("no VRR")
+    for r in vrrs:
+        X = names.index(r.changing_index().name())
+        for _ in range(trials):
+            ctx, ex, cen, C = ctx_c()
+            target = {n: tuple(int(v) for v in rng.integers(0, 3, size=3)) for n in names}
+            i = int(rng.integers(0, 3))
+            if target[names[X]][i] == 0:
+                t = list(target[names[X]]); t[i] = 1; target[names[X]] = tuple(t)
+            ctx.update(dir=i, index=target, aux={mname: int(rng.integers(0, 3))} if mname else {})
+            got = _lin_clean(r.expr().lin(ctx))
+            ref = _expected_onee_vrr(kind, ex, cen, names, mname, X, target, i, C)
+            for k in set(got) | set(ref):
+                g, f = got.get(k, 0.0), ref.get(k, 0.0)
+                if not abs(g - f) <= 1e-10 * max(abs(f), 1e-3):
+                    raise NotImplementedError("VRR %s on %s is not the Obara-Saika %s recurrence (term %s: %r vs %r)"
+                                              % (r.name(), names[X], kind, k, g, f))
+    built = {r.changing_index().name() for r in vrrs}
+    for r in hrrs:
+        built.add(r.changing_index().name())
+    if built != set(names):
+        raise NotImplementedError("every index must be built by a VRR or the HRR (built: %s)" % sorted(built))
+    return kind
+
+
+# ---------------------------------------------------------------------------
 # bindings
 # ---------------------------------------------------------------------------
 class EriBinding:
@@ -302,6 +421,84 @@ class EriBinding:
             "#endif", ""])
 
 
+class OneElectronBinding:
+    """A 2-index overlap or nuclear-attraction integral bound to libintc's
+    one-electron kernels (shells up to l = 3)."""
+
+    def __init__(self, integral, kind, options):
+        self.integral = integral
+        self.kind = kind
+        self.options = options
+        self.full_name = integral.full_name() + options.suffix()
+        self.spherical = options.spherical_transformed()
+        self.contracted = options.contracted()
+
+    def _fn(self):
+        from . import lib
+        L = lib.load()
+        lib.check(L.intc_set_boys_mode(1 if self.options.boys() == "reference" else 0))
+        pre = "intc_overlap_" if self.kind == "overlap" else "intc_nuclear_"
+        if not self.contracted:
+            return getattr(L, pre + "prim_cart_main")
+        return getattr(L, pre + ("cont_sph_main" if self.spherical else "cont_cart_main"))
+
+    def main(self, shells, output_array, C=None, strides=None, cont_strides=None):
+        """Per-call twin of <full>_main.  Primitive (contracted=False): shells
+        are 2 dicts with a scalar 'exps', 'cen', 'l'; contracted: as for the
+        ERI binding (exps, coefs n_prim x n_cont, cen, l, sph when spherical).
+        C: the point charge (nuclear attraction)."""
+        import ctypes
+        from . import lib
+        from .basis import ncc, nsph
+        fn = self._fn()
+        dp = ctypes.POINTER(ctypes.c_double)
+        ns = [nsph(s["l"]) if self.spherical else ncc(s["l"]) for s in shells]
+        strides = strides or [ns[1], 1]
+        cont_strides = cont_strides or [0, 0]
+        keep, args = [], []
+        for k, s in enumerate(shells):
+            cen = np.ascontiguousarray(s["cen"], dtype=np.float64)
+            keep.append(cen)
+            if not self.contracted:
+                args += [float(np.asarray(s["exps"]).reshape(-1)[0]), cen.ctypes.data_as(dp), s["l"], ns[k],
+                         int(strides[k])]
+                continue
+            x = np.ascontiguousarray(s["exps"], dtype=np.float64)
+            c = np.asfortranarray(np.asarray(s["coefs"], dtype=np.float64).reshape(len(x), -1))
+            keep += [x, c]
+            args += [x.ctypes.data_as(dp), cen.ctypes.data_as(dp), s["l"], ns[k], int(strides[k]), len(x),
+                     c.shape[1], int(cont_strides[k]), c.ctypes.data_as(dp)]
+            if self.spherical:
+                S = np.asfortranarray(s["sph"], dtype=np.float64)
+                keep.append(S)
+                args.append(S.ctypes.data_as(dp))
+        if self.kind == "nuclear":
+            cc = np.ascontiguousarray(C, dtype=np.float64)
+            keep.append(cc)
+            args.append(cc.ctypes.data_as(dp))
+        work = np.zeros(1)
+        lib.check(fn(*args, work.ctypes.data_as(dp), output_array.ctypes.data_as(dp)))
+        return output_array
+
+    def plan(self, shells, items, centres=None, device="cuda"):
+        from .runtime import OneEPlan
+        self._fn()
+        return OneEPlan(0 if self.kind == "overlap" else 1, shells, items, centres=centres,
+                        spherical=self.spherical, device=device)
+
+    def header(self):
+        n = self.full_name
+        pre = "intc_overlap_" if self.kind == "overlap" else "intc_nuclear_"
+        fn = pre + ("prim_cart_main" if not self.contracted else
+                    "cont_sph_main" if self.spherical else "cont_cart_main")
+        return "\n".join([
+            "/* Generated by intception_amd.generator (hip backend) for integral '%s'. */" % self.integral.name(),
+            "#ifndef %s_H" % n.upper(), "#define %s_H" % n.upper(), '#include "intc.h"',
+            "/* Reference name mapped onto the libintc C-ABI; the function returns an",
+            " * int status (0 = success) where the reference's returns void. */",
+            "#define %s_main %s" % (n, fn), "#endif", ""])
+
+
 class generator:
     def __init__(self, *integrals, opt=None):
         self._opt = opt or generator_options()
@@ -309,10 +506,13 @@ class generator:
         for itg in integrals:
             if not isinstance(itg, dsl.dsl_integral):
                 raise TypeError("generator takes dsl_integral objects")
-            if not self._opt.contracted():
-                raise NotImplementedError("hip backend generates contracted integrals")
-            recognize_eri(itg)
-            b = EriBinding(itg, self._opt)
+            if len(itg.index_list()) == 2:
+                b = OneElectronBinding(itg, recognize_onee(itg), self._opt)
+            else:
+                if not self._opt.contracted():
+                    raise NotImplementedError("hip backend generates contracted ERIs")
+                recognize_eri(itg)
+                b = EriBinding(itg, self._opt)
             self._bindings[b.full_name] = b
 
     def generator_options(self):
@@ -334,7 +534,8 @@ class generator:
         for n, b in self._bindings.items():
             with open(os.path.join(d, n + ".h"), "w") as fh:
                 fh.write(b.header())
-            manifest[n] = {"integral": b.integral.name(), "spherical": b.spherical, "backend": "hip",
+            manifest[n] = {"integral": b.integral.name(), "kind": getattr(b, "kind", "eri"),
+                           "spherical": b.spherical, "backend": "hip",
                            "precision": self._opt.precision(), "boys": self._opt.boys(),
                            "library": "libintc.so", "header": "intc.h"}
         with open(os.path.join(d, "intception_hip_manifest.json"), "w") as fh:

@@ -49,6 +49,18 @@ SIGNATURES = [
     ("intc_plan_route_counts", c_int, [c_void_p, _ip]),
     ("intc_plan_quartet_order", c_int, [c_void_p, _ip]),
     ("intc_plan_destroy", c_int, [c_void_p]),
+    ("intc_overlap_prim_cart_main", c_int, [c_double, _dp, c_int, c_int, c_int, c_double, _dp, c_int, c_int, c_int,
+                                            _dp, _dp]),
+    ("intc_nuclear_prim_cart_main", c_int, [c_double, _dp, c_int, c_int, c_int, c_double, _dp, c_int, c_int, c_int,
+                                            _dp, _dp, _dp]),
+    ("intc_overlap_cont_cart_main", c_int, _SHELL * 2 + [_dp, _dp]),
+    ("intc_overlap_cont_sph_main", c_int, (_SHELL + [_dp]) * 2 + [_dp, _dp]),
+    ("intc_nuclear_cont_cart_main", c_int, _SHELL * 2 + [_dp, _dp, _dp]),
+    ("intc_nuclear_cont_sph_main", c_int, (_SHELL + [_dp]) * 2 + [_dp, _dp, _dp]),
+    ("intc_onee_plan_create", c_int, [ctypes.POINTER(c_void_p), c_int, c_int, c_int, _ip, _ip, _ip, _dp, _dp, _dp,
+                                      c_int, c_ll, _ip, _llp]),
+    ("intc_onee_plan_execute", c_int, [c_void_p] * 8),
+    ("intc_onee_plan_destroy", c_int, [c_void_p]),
 ]
 
 

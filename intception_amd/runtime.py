@@ -192,6 +192,75 @@ class EriPlan:
             pass
 
 
+class OneEPlan:
+    """Batched one-electron integrals: kind 0 overlap (items = shell pairs
+    (a, b)), kind 1 nuclear attraction (items = (a, b, c), c indexing
+    `centres`, the point charges).  shells: any object with shell_l,
+    shell_nprim, shell_off, shell_cen, exps and coefs (e.g. a QuartetBatch or
+    a Molecule); blocks are row-major [a][b], dense in item order unless
+    out_off is given.  sph: None = built-in real solid harmonics."""
+
+    def __init__(self, kind, shells, items, centres=None, spherical=True, out_off=None, device="cuda", sph=None):
+        torch = _torch()
+        L = _lib.load()
+        self._L = L
+        self.kind = int(kind)
+        self.device = _cuda_device(torch, device)
+        items = np.ascontiguousarray(items, dtype=np.int32)
+        w = 2 if self.kind == 0 else 3
+        assert items.ndim == 2 and items.shape[1] == w
+        sl = np.ascontiguousarray(shells.shell_l, dtype=np.int32)
+        snp = np.ascontiguousarray(shells.shell_nprim, dtype=np.int32)
+        soff = np.ascontiguousarray(shells.shell_off, dtype=np.int32)
+        nc = np.array([(2 * l + 1) if spherical else (l + 1) * (l + 2) // 2 for l in range(8)], dtype=np.int64)
+        self.sizes = nc[sl[items[:, 0]]] * nc[sl[items[:, 1]]]
+        if out_off is None:
+            out_off = np.zeros(len(items), dtype=np.int64)
+            if len(items):
+                out_off[1:] = np.cumsum(self.sizes)[:-1]
+        self.out_off = np.ascontiguousarray(out_off, dtype=np.int64)
+        self.out_size = int((self.out_off + self.sizes).max()) if len(items) else 0
+        n_centre = 0 if centres is None else len(centres)
+        ip = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+        h_cen = np.ascontiguousarray(shells.shell_cen, dtype=np.float64).reshape(-1)
+        h_ex = np.ascontiguousarray(shells.exps, dtype=np.float64)
+        h_co = np.ascontiguousarray(shells.coefs, dtype=np.float64)
+        dp = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        h = ctypes.c_void_p()
+        with torch.cuda.device(self.device):
+            _lib.check(L.intc_onee_plan_create(ctypes.byref(h), self.kind, int(bool(spherical)), len(sl), ip(sl),
+                                               ip(snp), ip(soff), dp(h_cen), dp(h_ex), dp(h_co), n_centre,
+                                               len(items), ip(items),
+                                               self.out_off.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong))))
+        self.handle = h
+        f64 = lambda a: torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64).reshape(-1)).to(self.device)
+        self.d_exps, self.d_coefs, self.d_cen = f64(shells.exps), f64(shells.coefs), f64(shells.shell_cen)
+        self.d_centres = f64(centres) if centres is not None else None
+        self.d_sph = f64(sph) if sph is not None else None
+
+    def run(self, out=None):
+        torch = _torch()
+        if out is None:
+            out = torch.empty(self.out_size, dtype=torch.float64, device=self.device)
+        c = _ptr(self.d_centres) if self.d_centres is not None else ctypes.c_void_p()
+        sph = _ptr(self.d_sph) if self.d_sph is not None else ctypes.c_void_p()
+        st = ctypes.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
+        _lib.check(self._L.intc_onee_plan_execute(self.handle, _ptr(self.d_exps), _ptr(self.d_coefs),
+                                                  _ptr(self.d_cen), c, sph, _ptr(out), st))
+        return out
+
+    def close(self):
+        if self.handle:
+            self._L.intc_onee_plan_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def set_boys_mode(mode):
     _lib.check(_lib.load().intc_set_boys_mode(int(mode)))
 

@@ -179,6 +179,9 @@ static void boys_eval(double *f, int nmax, double x)
         oracle_boys_exact(f, 0, nmax, x);
 }
 
+/* F_0..F_nmax in the current mode (shared with onee_oracle.c) */
+void oracle_boys_eval(double *f, int nmax, double x) { boys_eval(f, nmax, x); }
+
 /* ------------------------------------------------------------------ */
 /* work_array_size                                                     */
 /* ------------------------------------------------------------------ */

@@ -16,6 +16,7 @@ void oracle_set_boys_mode(int mode);
 int oracle_get_boys_mode(void);
 void oracle_boys_ref(double *f, int ifskip, int nmin, int nmax, double x);
 void oracle_boys_exact(double *f, int nmin, int nmax, double x);
+void oracle_boys_eval(double *f, int nmax, double x);
 
 int oracle_eri_work_array_size(int spherical, int la, int lb, int lc, int ld,
                                int na_prim, int nb_prim, int nc_prim, int nd_prim,
@@ -40,6 +41,14 @@ int oracle_eri_batch(int spherical, int n_shell, const int *shell_l, const int *
                      const double *coefs, const double *const *sph_by_l,
                      long n_quartet, const int *quartets, const long *out_off, double *out,
                      int nthreads);
+
+/* one-electron kinds (onee_oracle.c): kind 0 overlap, 1 nuclear attraction */
+void oracle_onee_prim(int kind, int la, int lb, double xa, const double ca[3], double xb, const double cb[3],
+                      const double cc[3], double *out);
+int oracle_onee_batch(int kind, int spherical, int n_shell, const int *shell_l, const int *shell_nprim,
+                      const int *shell_off, const double *shell_cen, const double *exps, const double *coefs,
+                      const double *const *sph_by_l, const double *centres, long n_item, const int *items,
+                      const long *out_off, double *out, int nthreads);
 
 #ifdef __cplusplus
 }

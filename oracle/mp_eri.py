@@ -1,4 +1,5 @@
-"""Independent arbitrary-precision ERI (McMurchie-Davidson) with mpmath.
+"""Independent arbitrary-precision ERI, overlap and nuclear attraction
+integrals (McMurchie-Davidson) with mpmath.
 
 TEST INFRASTRUCTURE ONLY.  This is a different algorithm from the reference's
 Obara-Saika/Head-Gordon-Pople scheme: Hermite expansion coefficients E_t^{ij}
@@ -109,6 +110,33 @@ def primitive_eri(ca, cb, cc, cd, a, b, c, d, A, B, C, D):
             e2 = Ecd[0][tau] * Ecd[1][nu] * Ecd[2][phi]
             s += e1 * e2 * (-1) ** (tau + nu + phi) * R(t + tau, u + nu, v + phi)
     return 2 * mp.pi ** mp.mpf(2.5) / (p * q * mp.sqrt(p + q)) * s
+
+
+def primitive_overlap(ca, cb, a, b, A, B):
+    """(a|b) for unnormalised primitive Cartesian Gaussians: per direction
+    S_ij = E_0^{ij} sqrt(pi/p) (Helgaker et al. eq. 9.5.41)."""
+    A = [mp.mpf(v) for v in A]; B = [mp.mpf(v) for v in B]
+    a, b = mp.mpf(a), mp.mpf(b)
+    p = a + b
+    s = mp.mpf(1)
+    for k in range(3):
+        s *= _hermite_E(ca[k], cb[k], a, b, A[k] - B[k])[0] * mp.sqrt(mp.pi / p)
+    return s
+
+
+def primitive_nuclear(ca, cb, a, b, A, B, C):
+    """(a|1/r_C|b) for unnormalised primitive Cartesian Gaussians:
+    2 pi/p sum_tuv E_t E_u E_v R_tuv(p, P - C) (Helgaker et al. eq. 9.9.32)."""
+    A = [mp.mpf(v) for v in A]; B = [mp.mpf(v) for v in B]; C = [mp.mpf(v) for v in C]
+    a, b = mp.mpf(a), mp.mpf(b)
+    p = a + b
+    P = [(a * A[k] + b * B[k]) / p for k in range(3)]
+    E = [_hermite_E(ca[k], cb[k], a, b, A[k] - B[k]) for k in range(3)]
+    R = _hermite_R(sum(ca) + sum(cb), p, [P[k] - C[k] for k in range(3)])
+    s = mp.mpf(0)
+    for t, u, v in itertools.product(range(len(E[0])), range(len(E[1])), range(len(E[2]))):
+        s += E[0][t] * E[1][u] * E[2][v] * R(t, u, v)
+    return 2 * mp.pi / p * s
 
 
 def contracted_eri(shells, spherical=False, sph=None):

@@ -51,6 +51,11 @@ def lib():
     L.oracle_eri_batch.argtypes = [ctypes.c_int, ctypes.c_int, _ip, _ip, _ip, _dp, _dp, _dp,
                                    ctypes.POINTER(_dp), ctypes.c_long, _ip, _lp, _dp, ctypes.c_int]
     L.oracle_eri_batch.restype = ctypes.c_int
+    L.oracle_onee_prim.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, _dp, ctypes.c_double,
+                                   _dp, _dp, _dp]
+    L.oracle_onee_batch.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _ip, _ip, _ip, _dp, _dp, _dp,
+                                    ctypes.POINTER(_dp), _dp, ctypes.c_long, _ip, _lp, _dp, ctypes.c_int]
+    L.oracle_onee_batch.restype = ctypes.c_int
     t = np.load(_GOLDEN)
     small = np.ascontiguousarray(t["smallx"], dtype=np.float64)
     medium = np.ascontiguousarray(t["mediumx"], dtype=np.float64)
@@ -148,5 +153,45 @@ def eri_batch(batch, spherical, sph_by_l=None, nthreads=None):
                             soff.ctypes.data_as(_ip), _ptr(cen), _ptr(ex), _ptr(co), sph_list,
                             len(oo), q.ctypes.data_as(_ip), oo.ctypes.data_as(_lp), _ptr(out),
                             int(nthreads))
+    assert rc == 0
+    return out
+
+
+def onee_prim(kind, la, lb, xa, A, xb, B, C=None):
+    """Primitive Cartesian one-electron block (kind 0 overlap, 1 nuclear
+    attraction at C), row-major [a][b] (onee_oracle.c)."""
+    out = np.zeros(((la + 1) * (la + 2) // 2) * ((lb + 1) * (lb + 2) // 2))
+    arr = [np.ascontiguousarray(v, dtype=np.float64) for v in (A, B, C if C is not None else np.zeros(3))]
+    lib().oracle_onee_prim(int(kind), int(la), int(lb), float(xa), _ptr(arr[0]), float(xb), _ptr(arr[1]),
+                           _ptr(arr[2]), _ptr(out))
+    return out
+
+
+def onee_batch(kind, spherical, shells, items, out_off, out_size, centres=None, sph_by_l=None, nthreads=None):
+    """Contracted one-electron blocks over an SoA shell table.  shells: object
+    with shell_l, shell_nprim, shell_off, shell_cen, exps, coefs; items (n, 2)
+    shell pairs or (n, 3) (a, b, centre); centres (n_centre, 3)."""
+    L = lib()
+    nthreads = nthreads or (os.cpu_count() or 1)
+    out = np.zeros(int(out_size))
+    sph_list = (_dp * 8)()
+    keep = []
+    if spherical:
+        for l in range(8):
+            if l in sph_by_l:
+                S = np.asfortranarray(sph_by_l[l], dtype=np.float64)
+                keep.append(S)
+                sph_list[l] = S.ctypes.data_as(_dp)
+    ip = lambda a: np.ascontiguousarray(a, dtype=np.int32)
+    sl, snp, soff = ip(shells.shell_l), ip(shells.shell_nprim), ip(shells.shell_off)
+    it = ip(np.asarray(items).reshape(-1))
+    oo = np.ascontiguousarray(out_off, dtype=np.int64)
+    cen = np.ascontiguousarray(shells.shell_cen, dtype=np.float64)
+    ex = np.ascontiguousarray(shells.exps, dtype=np.float64)
+    co = np.ascontiguousarray(shells.coefs, dtype=np.float64)
+    cc = np.ascontiguousarray(centres if centres is not None else np.zeros((1, 3)), dtype=np.float64)
+    rc = L.oracle_onee_batch(int(kind), int(spherical), len(sl), sl.ctypes.data_as(_ip), snp.ctypes.data_as(_ip),
+                             soff.ctypes.data_as(_ip), _ptr(cen), _ptr(ex), _ptr(co), sph_list, _ptr(cc),
+                             len(oo), it.ctypes.data_as(_ip), oo.ctypes.data_as(_lp), _ptr(out), int(nthreads))
     assert rc == 0
     return out

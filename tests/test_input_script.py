@@ -118,3 +118,43 @@ def test_bound_main_matches_oracle(gpu, sph):
         b.main(shells, out, strides=strides, cont_strides=cstr)
         ref = oracle.eri_quartet(shells, sph, basis.SPH, strides=strides, ncont_strides=cstr, out_size=16 * blk)
         assert_parity(out, ref, what="bound main %s sph=%s" % (cls, sph))
+
+
+def test_onee_scripts_recognised(tmp_path):
+    """The example script's integral kinds: the 2-index overlap and nuclear
+    attraction (VRR-only, and the nuclear attraction with the HRR) are
+    recognised, a dropped -(P-C) term and a wrong base are rejected, and the
+    generator maps the reference names onto the one-electron C-ABI."""
+    import json
+    from intception_amd import onee_script
+    from intception_amd import dsl as d
+    from intception_amd.generator import recognize_onee
+    assert recognize_onee(onee_script.make_overlap()) == "overlap"
+    assert recognize_onee(onee_script.make_nuclear()) == "nuclear"
+    assert recognize_onee(onee_script.make_nuclear(hrr=True)) == "nuclear"
+    good = onee_script.make_nuclear()
+    bad = d.dsl_integral(*good.index_list(), *good.aux_list(), *good.arg_list(), name="nuclear_2idx")
+    bad.set_base(good.base())
+    ga, gb = good.index_list()
+    m = good.aux_list()[0]
+    vb, va = good.rr_list()
+    bad.add_vrr("vrr_b", gb, vb.expr())
+    # drop -(P-C)_i (a-1|b)^(m+1): the overlap recurrence in the nuclear integral
+    PC = onee_script._common()[4]["PC"]
+    bad.add_vrr("vrr_a", ga, va.expr() + PC * bad.int(ga - 1, gb, m + 1))
+    with pytest.raises(NotImplementedError, match="not the Obara-Saika nuclear"):
+        recognize_onee(bad)
+    wrong = onee_script.make_overlap()
+    wrong.set_base(wrong.base() * 2.0)
+    with pytest.raises(NotImplementedError):
+        recognize_onee(wrong)
+    gen = generator(onee_script.make_overlap(), onee_script.make_nuclear(),
+                    opt=generator_options(output_directory=str(tmp_path), contracted=False,
+                                          spherical_transformed=False))
+    man = gen.out()
+    assert set(man) == {"intception_integral_overlap_2idx_ab_prim_cart",
+                        "intception_integral_nuclear_2idx_ab_m_cc_prim_cart"}
+    hdr = open(tmp_path / "intception_integral_nuclear_2idx_ab_m_cc_prim_cart.h").read()
+    assert "#define intception_integral_nuclear_2idx_ab_m_cc_prim_cart_main intc_nuclear_prim_cart_main" in hdr
+    assert json.load(open(tmp_path / "intception_hip_manifest.json"))[
+        "intception_integral_overlap_2idx_ab_prim_cart"]["kind"] == "overlap"
