@@ -490,6 +490,114 @@ def run_config1(args):
     return res
 
 
+def run_onee(args):
+    """One-electron kinds of the input-script API on the config-3 molecule:
+    the overlap over every shell pair a <= b and the nuclear attraction of
+    every Schwarz-prescreened pair at every atom, spherical, FP64."""
+    import torch
+    from intception_amd import flops_model, lib, molecule, runtime
+    from intception_amd.basis import SPH
+    lib.check(lib.load().intc_init())
+    mol = molecule.Molecule(args.atoms or 200, 202603)
+    pairs_all = mol.all_pairs()
+    pre = mol.prescreen_pairs()
+    n_atoms = len(mol.atoms)
+    items_v = np.concatenate([np.repeat(pre, n_atoms, axis=0),
+                              np.tile(np.arange(n_atoms, dtype=np.int32), len(pre))[:, None]], axis=1)
+    t0 = time.perf_counter()
+    plan_s = runtime.OneEPlan(0, mol, pairs_all, spherical=True)
+    plan_v = runtime.OneEPlan(1, mol, items_v, centres=mol.atoms, spherical=True)
+    log("onee: %d overlap pairs, %d nuclear-attraction items (%d pairs x %d atoms), plans %.1fs"
+        % (len(pairs_all), len(items_v), len(pre), n_atoms, time.perf_counter() - t0))
+    dev = plan_s.device
+    stream = torch.cuda.current_stream(dev)
+    out_s = torch.empty(plan_s.out_size, dtype=torch.float64, device=dev)
+    out_v = torch.empty(plan_v.out_size, dtype=torch.float64, device=dev)
+    for _ in range(args.warmup):
+        plan_s.run(out_s)
+        plan_v.run(out_v)
+    ms_s = _timed(torch, dev, stream, lambda: plan_s.run(out_s), args.steps)
+    ms_v = _timed(torch, dev, stream, lambda: plan_v.run(out_v), args.steps)
+    log("overlap %.3f ms, nuclear attraction %.3f ms" % (ms_s, ms_v))
+    # algorithmic work and bytes
+    ls, nps = mol.shell_l, mol.shell_nprim
+
+    def work(kind, items):
+        la, lb = ls[items[:, 0]], ls[items[:, 1]]
+        pp = nps[items[:, 0]].astype(np.int64) * nps[items[:, 1]]
+        tot = 0.0
+        for a in range(3):
+            for b in range(3):
+                for k in np.unique(pp):
+                    sel = (la == a) & (lb == b) & (pp == k)
+                    n = int(sel.sum())
+                    if n:
+                        tot += n * flops_model.onee_flops(kind, a, b, int(k), True)
+        return tot
+    f_v = work(1, items_v)
+    bytes_s = 8.0 * plan_s.out_size + 16.0 * len(pairs_all)
+    # parity spot check against the oracle
+    from oracle import oracle
+    rng = np.random.default_rng(3)
+    check = {}
+    for kind, plan, out, items in ((0, plan_s, out_s, pairs_all), (1, plan_v, out_v, items_v)):
+        idx = np.sort(rng.choice(len(items), size=min(2000, len(items)), replace=False))
+        sz = plan.sizes[idx]
+        off = np.concatenate([[0], np.cumsum(sz)[:-1]])
+        oracle.set_boys_mode(oracle.BOYS_EXACT)
+        ref = oracle.onee_batch(kind, True, mol, items[idx], off, int(sz.sum()), mol.atoms, SPH,
+                                min(16, os.cpu_count() or 1))
+        host = out.cpu().numpy()
+        got = np.concatenate([host[plan.out_off[q]:plan.out_off[q] + plan.sizes[q]] for q in idx])
+        err = np.abs(got - ref)
+        check["overlap" if kind == 0 else "nuclear"] = {"items": int(len(idx)), "max_abs_err": float(err.max())}
+    log("parity:", check)
+    cpu = None
+    if not args.no_cpu:
+        # the oracle on host cores over random items of the nuclear-attraction set
+        nthr = min(16, os.cpu_count() or 1)
+        done, t_cpu, n_int = 0, 0.0, 0
+        while t_cpu < 8.0 and done < len(items_v):
+            idx = rng.choice(len(items_v), size=200_000, replace=False)
+            sz = plan_v.sizes[idx]
+            off = np.concatenate([[0], np.cumsum(sz)[:-1]])
+            t0 = time.perf_counter()
+            oracle.onee_batch(1, True, mol, items_v[idx], off, int(sz.sum()), mol.atoms, SPH, nthr)
+            t_cpu += time.perf_counter() - t0
+            done += len(idx)
+            n_int += int(sz.sum())
+        cpu = {"value": n_int / t_cpu, "unit": "integrals/s", "cores": nthr, "kind": "port",
+               "sample": "%d nuclear-attraction items drawn at random from the same workload (%.1f s of CPU "
+                         "work); oracle/onee_oracle.c, gcc -O3 -fopenmp" % (done, t_cpu)}
+        log("cpu baseline:", cpu)
+    n_int = plan_s.out_size + plan_v.out_size
+    ms = ms_s + ms_v
+    res = {
+        "metric": "one-electron integrals/s on 1 MI355X (overlap + nuclear attraction, FP64)",
+        "value": float(n_int / (ms * 1e-3)), "unit": "integrals/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": float(ms), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "overlap over all %d shell pairs and nuclear attraction of %d prescreened pairs at "
+                               "each of %d atoms, config-3 molecule (seed 202603), spherical"
+                               % (len(pairs_all), len(pre), n_atoms),
+                   "integrals": int(n_int)},
+        "overlap": {"ms": float(ms_s), "integrals_per_s": float(plan_s.out_size / (ms_s * 1e-3)),
+                    "roofline": {"bound": "hbm", "achieved": bytes_s / (ms_s * 1e-3) / 1e9, "peak": 8000.0,
+                                 "unit": "GB/s", "frac": bytes_s / (ms_s * 1e-3) / 1e9 / 8000.0,
+                                 "note": "output bytes + 16 B item descriptor"}},
+        "nuclear_attraction": {"ms": float(ms_v), "integrals_per_s": float(plan_v.out_size / (ms_v * 1e-3)),
+                               "roofline": {"bound": "mfma", "achieved": f_v / (ms_v * 1e-3) / 1e12,
+                                            "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                                            "frac": f_v / (ms_v * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,
+                                            "note": "flops = flops_model.onee_flops (VRR on a, HRR, transforms)"}},
+        "parity": check,
+        "cpu_baseline": cpu,
+    }
+    plan_s.close()
+    plan_v.close()
+    return res
+
+
 def traffic_probe(args):
     """Counter-run helper: builds the config-3 plan, runs the plain path once,
     then launches the FP64 and the FP32-recurrence kernel of the class
@@ -528,7 +636,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", choices=["config3", "config1", "config4"], default="config3")
+    ap.add_argument("--workload", choices=["config3", "config1", "config4", "onee"], default="config3")
     ap.add_argument("--traffic-probe", action="store_true", help="counter-run helper (config-3 plan)")
     ap.add_argument("--probe-class", default="1121", help="class of the --traffic-probe kernel")
     ap.add_argument("--quartets", type=int, default=1_000_000, help="config1 size")
@@ -548,7 +656,8 @@ def main():
     if args.traffic_probe:
         res = traffic_probe(args)
     else:
-        res = {"config3": run_config3, "config4": run_config4, "config1": run_config1}[args.workload](args)
+        res = {"config3": run_config3, "config4": run_config4, "config1": run_config1,
+               "onee": run_onee}[args.workload](args)
     print(json.dumps(res), flush=True)
 
 

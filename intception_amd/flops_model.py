@@ -122,3 +122,41 @@ def batch_flops(batch, idx=None):
 def batch_bytes(batch, idx=None):
     sz = batch.sizes() if idx is None else batch.sizes()[idx]
     return 8.0 * sz + 16.0
+
+
+# ---------------------------------------------------------------------------
+# one-electron kinds (overlap, nuclear attraction)
+# ---------------------------------------------------------------------------
+@lru_cache(maxsize=None)
+def _onee_parts(kind, la, lb, spherical):
+    """(per primitive pair, per contracted item) flops of the one-electron
+    scheme: VRR (e|0)^(m) on a for |e| <= la + lb with the direction rule
+    (nuclear attraction: PA V^m - PC V^(m+1) [+ k/2zeta (V^m - V^(m+1))];
+    overlap: PA V [+ k/2zeta V]), accumulation of la <= |e|, then the HRR to b
+    and the transforms on the contracted block.  Setup per primitive pair:
+    zeta, P, PA, the Gaussian-product prefactor (30); Boys as for the ERI
+    (14 + 8 + 4M) and M + 1 prefactor products for the nuclear attraction."""
+    L = la + lb
+    nuc = kind == 1
+    M = L if nuc else 0
+    vrr = 0
+    for n in range(1, L + 1):
+        for e in cart_components(n):
+            i = _fnz(e)
+            k = e[i] - 1
+            for _ in (range(M - n + 1) if nuc else range(1)):
+                vrr += (3 if nuc else 1) + ((3 if nuc else 2) if k > 0 else 0)
+    setup = 30 + ((14 + 8 + 4 * M) + (M + 1) if nuc else 4)
+    acc = ncc0(L) - ncc0(la - 1)
+    hrr = sum(2 * ncc(bl) * (ncc0(L - bl) - ncc0(la - 1)) for bl in range(1, lb + 1))
+    if spherical:
+        tr = 2 * ncc(la) * ncc(lb) * nsph(lb) + 2 * ncc(la) * nsph(la) * nsph(lb)
+    else:
+        tr = 0
+    return setup + vrr + acc, hrr + tr
+
+
+def onee_flops(kind, la, lb, nprim_pairs, spherical):
+    """Algorithmic flops of one contracted one-electron item."""
+    p, c = _onee_parts(int(kind), int(la), int(lb), bool(spherical))
+    return nprim_pairs * p + c
