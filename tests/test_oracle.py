@@ -169,3 +169,20 @@ def test_oracle_three_centre_conditioning():
         lim = 1e-14 + 1e-12 * np.abs(ref) + 5e-11 * np.abs(ref).max()
         worst = max(worst, float(np.max(np.abs(got - ref) / lim)))
     assert worst <= 1.0
+
+
+def test_work_array_size_matches_reference_generated():
+    """<full>_work_array_size of the reference's own generated C for the ERI
+    script (tests/golden/make_work_array_golden.py: 1296 values over every
+    (la lb|lc ld) with l <= 2, Cartesian and spherical, four primitive-count
+    and two contraction-count patterns) equals the oracle's formula and
+    libintc's intc_eri_cont_{sph,cart}_work_array_size exactly."""
+    import os
+    from intception_amd import lib
+    z = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "work_array_size_golden.npz"))
+    L = lib.load()
+    for r in z["rows"]:
+        sph, ls, ks, kc, ref = int(r[0]), r[1:5], r[5:9], r[9:13], int(r[13])
+        assert oracle.work_array_size(sph, ls, ks, kc) == ref, r
+        f = L.intc_eri_cont_sph_work_array_size if sph else L.intc_eri_cont_cart_work_array_size
+        assert f(*[int(v) for v in ls], *[int(v) for v in ks], *[int(v) for v in kc]) == ref, r
