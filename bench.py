@@ -230,6 +230,26 @@ def run_molecule(args, three_center=False):
     log("router kernel: %.3f ms" % ms_route)
     learned_skip = run_learned(False)
     learned_mixed = run_learned(True)
+    # the method without the network, at the same skip threshold: Schwarz
+    # screening raised to tau_skip (|(ab|cd)| <= Q_P Q_P' makes every skipped
+    # block smaller than tau_skip), the rest in FP64
+    qq = mol.Q[pp[:, 0]] * mol.Q[pp[:, 1]]
+    route_s = torch.from_numpy((qq < tau_skip).astype(np.int8) * 2).to(dev)
+    del qq
+    plan.set_route_fp32(False)
+    plan.capture(outml, n_streams=args.streams, route=route_s)
+    for _ in range(args.warmup):
+        plan.replay()
+    ms_schwarz = _timed(torch, dev, stream, plan.replay, args.steps)
+    err_s = max(float((outml[s0:s0 + (1 << 28)] - out64[s0:s0 + (1 << 28)]).abs().max().item())
+                for s0 in range(0, batch.out_size, 1 << 28))
+    frac_s = float(route_s.to(torch.float32).mean().item() / 2.0)
+    del route_s
+    schwarz_only = {"ms_per_step": float(ms_schwarz), "integrals_per_s": float(batch.out_size / (ms_schwarz * 1e-3)),
+                    "skip_fraction": frac_s, "max_abs_err_vs_fp64": err_s,
+                    "rule": "skip when Q_P Q_P' < tau_skip = %.1e (Schwarz bound), FP64 otherwise" % tau_skip}
+    log("Schwarz screening at tau_skip (no network): %.3f ms/step, skips %.1f %% of quartets, max abs err %.3e"
+        % (ms_schwarz, 100 * frac_s, err_s))
     # the same two paths as ONE persistent multi-class launch (tiles of every
     # class bin from a device counter), captured in a graph
     plan.set_persistent(True)
@@ -294,6 +314,8 @@ def run_molecule(args, three_center=False):
                        "quartets_per_s": float(batch.n_quartet / (ms64 * 1e-3))},
         "learned_skip_fp64": learned_skip,
         "learned_skip_and_fp32": learned_mixed,
+        "schwarz_only_at_tau_skip": schwarz_only,
+        "speedup_vs_schwarz_only": float(ms_schwarz / msml),
         "persistent_launch": persistent,
         "value_from": ("learned_skip_and_fp32" if best is learned_mixed else
                        "learned_skip_fp64" if best is learned_skip else

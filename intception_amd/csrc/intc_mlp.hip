@@ -135,7 +135,8 @@ __global__ void __launch_bounds__(256) intc_mlp_features_k(long long n, const fl
 // The K index of a slot is the same function of (g, s) in A and B, so the
 // K chunks of layer 2 may order the hidden-1 units freely: chunk c, slot s
 // <-> unit 16 (2c + s / 4) + 4 g + s % 4, the unit layer 1's D holds there.
-// route: 2 if pred + m_skip < log_skip, else 1 if pred + m_32 < log_32 and
+// route: 2 if pred + m_skip < log_skip or log10 Q_bra + log10 Q_ket <
+// log_skip (the Schwarz bound alone), else 1 if pred + m_32 < log_32 and
 // log10 Q_bra + log10 Q_ket < log_qq32, else 0.
 // ---------------------------------------------------------------------------
 constexpr int SF = KF + 1;  // LDS row: 16 standardised features (zero padded), padded to 17 words
@@ -239,7 +240,10 @@ __global__ void __launch_bounds__(256) intc_mlp_route_k(long long n, const float
             if (pred) pred[q] = y;
             if (route) {
                 signed char r = 0;
-                if (y + m_skip < log_skip) r = 2;
+                // skip on the network's calibrated prediction, or on the
+                // Schwarz bound itself (|(ab|cd)| <= Q_bra Q_ket; 1e-4 decade
+                // guard for the FP32 logarithms)
+                if (y + m_skip < log_skip || lq < log_skip - 1e-4f) r = 2;
                 else if (y + m_32 < log_32 && lq < log_qq32) r = 1;
                 route[q] = r;
             }

@@ -100,6 +100,7 @@ class LearnedRouter:
         of them keeps its FP32-recurrence error within the FP32 budget."""
         m = self.net.margin_32
         keep = pred + self.net.margin_skip >= self.log_skip
+        keep &= logqq >= self.log_skip - 1e-4        # the route kernel also skips on the Schwarz bound
         best = (0, -99.0, -99.0)
         for tp in np.arange(-14.0, 2.01, 0.5):
             sel_p = keep & (pred + m < tp)
@@ -162,7 +163,10 @@ def build_router(n_train=2_000_000, train_seeds=(202611, 202612), val_seed=20261
              "val_max_underprediction": float(np.max(tv - pred)), "margin": margin,
              "val_fp32_max_err": float(e32.max()), "val_fp32_fraction": n32 / len(pred),
              "val_samples": int(len(pred)),
-             "val_skip_fraction": float(np.mean(pred + margin < np.log10(tau_skip))),
+             "val_skip_fraction": float(np.mean((pred + margin < np.log10(tau_skip)) |
+                                                (logqq < np.log10(tau_skip) - 1e-4))),
+             # the method without the network: Schwarz screening raised to tau_skip
+             "val_skip_schwarz_only": float(np.mean(logqq < np.log10(tau_skip))),
              # what a perfect predictor could skip: true magnitude below tau_skip
              "val_skip_ideal": float(np.mean(tv < np.log10(tau_skip))), **router.summary()}
     if log:

@@ -3,7 +3,8 @@
 A router trained on two small molecules of their own seeds and calibrated on a
 third routes the screened quartets of a fourth; the routed result stays within
 the 1e-10 absolute error budget of the plain FP64 result, and the router
-skips or demotes a share of the quartets that Schwarz screening alone keeps.
+skips more quartets than Schwarz screening raised to the same skip threshold
+(the method without the network) would.
 """
 import numpy as np
 import pytest
@@ -35,6 +36,19 @@ def test_learned_routing_small_molecule(gpu):
           flush=True)
     assert err <= 1e-10
     assert codes[2] + codes[1] > 0.05
+    # what the network gains over the method without it: Schwarz screening
+    # raised to the same skip threshold (the Schwarz bound makes that safe);
+    # the router skips on either test, so it never skips less, and the
+    # network adds the quartets whose bound is loose
+    qq = mol.Q[pp[:, 0]] * mol.Q[pp[:, 1]]
+    schwarz = qq < 10 ** router.log_skip
+    skipped = route.cpu().numpy() == 2
+    # clear of the kernel's 1e-4-decade guard for its FP32 logarithms
+    surely = np.log10(np.maximum(qq, 1e-300)) < router.log_skip - 2e-4
+    print("skip fraction: router %.3f, Schwarz at the same threshold %.3f, by the network only %.3f"
+          % (skipped.mean(), schwarz.mean(), (skipped & ~schwarz).mean()), flush=True)
+    assert np.all(skipped[surely])
+    assert (skipped & ~schwarz).mean() > 0.01
 
 
 def test_three_center_molecule(gpu):
