@@ -512,6 +512,104 @@ def run_config1(args):
     return res
 
 
+def run_config2(args):
+    """High angular momentum: 200 000 quartets over (dd|dd), (fd|fd), (ff|dd),
+    (ff|ff), K 1..3, spherical.  FP64 path and the FP32-recurrence path (FP64
+    Boys, prefactors, contraction sums, HRR and FP64-MFMA transforms), each as
+    one graph; the FP32 errors against the FP64 result are reported."""
+    import torch
+    from intception_amd import basis, flops_model, lib, runtime
+    from intception_amd.codegen import canonical_np
+    lib.check(lib.load().intc_init())
+    t0 = time.perf_counter()
+    batch = basis.config2(n_quartet=args.quartets2)
+    plan = runtime.EriPlan(batch)
+    log("config2: %d quartets, %d integrals, plan %.1fs" % (batch.n_quartet, batch.out_size,
+                                                            time.perf_counter() - t0))
+    dev = plan.device
+    stream = torch.cuda.current_stream(dev)
+    out64 = plan.alloc_out()
+    out32 = plan.alloc_out()
+    plan.run(out64)
+    plan.capture(out64, n_streams=args.streams)
+    for _ in range(args.warmup):
+        plan.replay()
+    ms64 = _timed(torch, dev, stream, plan.replay, args.steps)
+    route = torch.ones(batch.n_quartet, dtype=torch.int8, device=dev)
+    plan.set_route_fp32(True)
+    plan.capture(out32, n_streams=args.streams, route=route)
+    for _ in range(args.warmup):
+        plan.replay()
+    ms32 = _timed(torch, dev, stream, plan.replay, args.steps)
+    log("FP64 %.3f ms/step, FP32 recurrences %.3f ms/step" % (ms64, ms32))
+    h64 = out64.cpu().numpy()
+    h32 = out32.cpu().numpy()
+    err = np.abs(h32 - h64)
+    bmax = np.maximum.reduceat(np.abs(h64), batch.out_off)
+    emax = np.maximum.reduceat(err, batch.out_off)
+    canon = canonical_np(batch.classes())
+    rel_blk = emax / np.maximum(bmax, 1e-300)
+    per_class = {}
+    for c in sorted(set(map(tuple, canon))):
+        sel = np.all(canon == np.array(c), axis=1)
+        per_class["(%d%d|%d%d)" % c] = {"max_rel_to_block_max": float(rel_blk[sel].max()),
+                                        "median_rel_to_block_max": float(np.median(rel_blk[sel]))}
+    big = np.abs(h64) > 1e-8
+    fp32_err = {"max_abs": float(err.max()), "max_rel_elementwise_where_ref_gt_1e-8": float((err[big] /
+                                                                                           np.abs(h64[big])).max()),
+                "max_rel_to_block_max": float(rel_blk.max()), "per_class": per_class}
+    log("FP32 errors:", fp32_err)
+    del h32, err
+    # FP64 parity spot check against the oracle
+    from oracle import oracle
+    idx = np.arange(0, batch.n_quartet, max(1, batch.n_quartet // 200))[:200]
+    sub = batch.subset(idx)
+    oracle.set_boys_mode(oracle.BOYS_EXACT)
+    ref = oracle.eri_batch(sub, True, basis.SPH, min(16, os.cpu_count() or 1))
+    sz = batch.sizes()
+    got = np.concatenate([h64[batch.out_off[q]:batch.out_off[q] + sz[q]] for q in idx])
+    d = np.abs(got - ref)
+    outside = d > 1e-14 + 1e-12 * np.abs(ref)
+    sub_off = np.concatenate([[0], np.cumsum(sz[idx])[:-1]])
+    rel_b = np.maximum.reduceat(d, sub_off) / np.maximum(np.maximum.reduceat(np.abs(ref), sub_off), 1e-300)
+    check = {"quartets": int(len(idx)), "elements": int(len(ref)), "max_abs_err": float(d.max()),
+             "max_err_rel_to_block_max": float(rel_b.max()),
+             "elements_outside_1e-12_rel_1e-14_abs": int(outside.sum()),
+             "within_1e-12_rel_1e-14_abs": bool(not outside.any())}
+    log("FP64 parity check:", check)
+    del h64
+    flops_q = flops_model.batch_flops(batch)
+    per_bin = _per_bin(torch, plan, out64, flops_q, canon, stream)
+    dom = max(per_bin, key=lambda r: r["ms"])
+    achieved = dom["flops"] / (dom["ms"] * 1e-3) / 1e12
+    cpu = None if args.no_cpu else cpu_baseline(batch, 2000)
+    if cpu:
+        log("cpu baseline:", cpu)
+    n_int = batch.out_size
+    res = {
+        "metric": "contracted ERIs/s on 1 MI355X (config 2, high angular momentum; FP64 and FP32 recurrences)",
+        "value": float(n_int / (ms32 * 1e-3)), "unit": "integrals/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": float(ms32), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32 recurrences + f64 Boys/prefactors/contraction/HRR/transforms",
+        "data": "synthetic",
+        "config": {"workload": "config2: %d quartets, (dd|dd),(fd|fd),(ff|dd),(ff|ff), K 1..3, exponents "
+                               "10^U[log 0.05, log 50), spherical" % batch.n_quartet,
+                   "quartets": int(batch.n_quartet), "integrals": int(n_int), "seed": 202602},
+        "plain_fp64": {"ms_per_step": float(ms64), "integrals_per_s": float(n_int / (ms64 * 1e-3))},
+        "fp32_recurrences": {"ms_per_step": float(ms32), "integrals_per_s": float(n_int / (ms32 * 1e-3)),
+                             "errors_vs_fp64": fp32_err},
+        "parity_fp64_vs_oracle": check,
+        "roofline": {"bound": "mfma", "achieved": float(achieved), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": float(achieved / FP64_PEAK_TFLOPS), "traffic": None,
+                     "kernel": kernel_label(dom), "kernel_ms": dom["ms"], "kernel_flops": dom["flops"],
+                     "note": "dominant FP64 bin; flops = reference-scheme count (intception_amd/flops_model.py)"},
+        "cpu_baseline": cpu,
+        "bins": [{"cls": r["cls"], "nq": r["nq"], "ms": round(r["ms"], 4)} for r in per_bin],
+    }
+    plan.close()
+    return res
+
+
 def run_onee(args):
     """One-electron kinds of the input-script API on the config-3 molecule:
     the overlap over every shell pair a <= b and the nuclear attraction of
@@ -658,10 +756,12 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", choices=["config3", "config1", "config4", "onee"], default="config3")
+    ap.add_argument("--workload", choices=["config3", "config1", "config2", "config4", "onee"],
+                    default="config3")
     ap.add_argument("--traffic-probe", action="store_true", help="counter-run helper (config-3 plan)")
     ap.add_argument("--probe-class", default="1121", help="class of the --traffic-probe kernel")
     ap.add_argument("--quartets", type=int, default=1_000_000, help="config1 size")
+    ap.add_argument("--quartets2", type=int, default=200_000, help="config2 size")
     ap.add_argument("--atoms", type=int, default=None, help="molecule size (config3 200, config4 100)")
     ap.add_argument("--train", type=int, default=2_000_000, help="config3 training quartets")
     ap.add_argument("--cpu-sample", type=int, default=150_000)
@@ -678,7 +778,7 @@ def main():
     if args.traffic_probe:
         res = traffic_probe(args)
     else:
-        res = {"config3": run_config3, "config4": run_config4, "config1": run_config1,
+        res = {"config3": run_config3, "config4": run_config4, "config1": run_config1, "config2": run_config2,
                "onee": run_onee}[args.workload](args)
     print(json.dumps(res), flush=True)
 
