@@ -45,7 +45,8 @@ class parallelism:
 class generator_options:
     """Reference options (contracted, spherical_transformed, global_user_lmax,
     use_cblas, output_directory, suffixes, silent) plus this build's:
-    backend ("hip"), precision ("fp64" or "mixed"), boys ("chebyshev" or
+    backend ("hip": bind to libintc's kernels; "c99": emit CPU C99 source of the
+    ERI class), precision ("fp64" or "mixed"), boys ("chebyshev" or
     "reference"), screening threshold."""
 
     def __init__(self, contracted=None, spherical_transformed=None, global_user_lmax=None, use_cblas=None,
@@ -64,8 +65,8 @@ class generator_options:
                         "sph": sph_suffix or "sph"}
         self._silent = bool(silent)
         self._backend = backend or "hip"
-        if self._backend != "hip":
-            raise ValueError("this build's generator has the 'hip' backend")
+        if self._backend not in ("hip", "c99"):
+            raise ValueError("this build's generator has the 'hip' and 'c99' backends")
         self._precision = precision or "fp64"
         assert self._precision in ("fp64", "mixed")
         self._boys = boys or "chebyshev"
@@ -530,6 +531,15 @@ class generator:
     def out(self):
         d = self._opt.output_directory()
         os.makedirs(d, exist_ok=True)
+        if self._opt.backend() == "c99":
+            # CPU source of each ERI class (<full>.c, <full>.h), reference interface
+            from . import c99gen
+            written = {}
+            for n, b in self._bindings.items():
+                if not isinstance(b, EriBinding):
+                    raise NotImplementedError("the c99 backend emits the ERI class only")
+                written[n] = c99gen.write(d, n, b.spherical, self._opt.global_user_lmax())
+            return written
         manifest = {}
         for n, b in self._bindings.items():
             with open(os.path.join(d, n + ".h"), "w") as fh:
