@@ -280,6 +280,31 @@ def run_molecule(args, three_center=False):
     dom = max(per_bin, key=lambda r: r["ms"])
     achieved = dom["flops"] / (dom["ms"] * 1e-3) / 1e12
     dom_name = kernel_label(dom)
+    # the FP32-recurrence twin of the dominant kernel, isolated, with its
+    # counter-measured FP32 / FP64 VALU FLOPs (committed counter run)
+    fp32_twin = None
+    # (classes with an f or g shell run their FP32 route on the FP64 kernel)
+    if dom.get("G") != 64 and max(dom["cls"]) < 3:
+        ms32 = _timed(torch, dev, stream, lambda: plan.run_bin(dom["bin"], out64, 1), 3)
+        name32 = dom_name.replace("intc_eri_d", "intc_eri_f", 1)
+        fp32_twin = {"kernel": name32, "kernel_ms": float(ms32), "fp64_kernel_ms": float(dom["ms"])}
+        try:
+            with open(os.path.join(ROOT, "profiles", "traffic.json")) as fh:
+                t32 = json.load(fh).get(name32)
+        except (OSError, ValueError):
+            t32 = None
+        if t32 and t32.get("workload") == ("config4" if three_center else "config3"):
+            f32, f64 = t32["valu_flops_fp32_per_launch"], t32["valu_flops_fp64_per_launch"]
+            fp32_twin.update({"fp32_tflops": f32 / (ms32 * 1e-3) / 1e12,
+                              "fp32_frac_of_157.3": f32 / (ms32 * 1e-3) / 1e12 / 157.3,
+                              "fp64_tflops": f64 / (ms32 * 1e-3) / 1e12, "source": t32["valu_flops_source"]})
+        # plan.run_bin wrote FP32 results into out64; the plain leg's checks are done
+    # the router's BF16 MFMA work: 12->64->64->1 per quartet (layer 1 padded to 16 inputs)
+    router_flops = float(batch.n_quartet) * 2.0 * (16 * 64 + 64 * 64 + 64)
+    router_mfma = {"flops_per_step": router_flops, "achieved": router_flops / (ms_route * 1e-3) / 1e12,
+                   "peak": 2500.0, "unit": "TFLOP/s",
+                   "frac": router_flops / (ms_route * 1e-3) / 1e12 / 2500.0,
+                   "note": "BF16 MFMA layers of the routing network over every quartet of the step"}
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(batch, args.cpu_sample)
@@ -321,6 +346,8 @@ def run_molecule(args, three_center=False):
                        "learned_skip_fp64" if best is learned_skip else
                        "plain_fp64 (no learned variant within 1e-10)"),
         "router_ms": float(ms_route),
+        "router_mfma": router_mfma,
+        "fp32_twin_of_dominant_kernel": fp32_twin,
         "speedup_vs_plain_fp64": float(ms64 / msml),
         "router": {k: v for k, v in rstats.items() if k != "train_mse"},
         "roofline": {"bound": "mfma", "achieved": float(achieved), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
