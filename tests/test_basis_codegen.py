@@ -130,3 +130,35 @@ def test_library_exports_every_declared_symbol():
     assert L.intc_eri_cont_cart_work_array_size(1, 1, 1, 1, 3, 3, 3, 3, 1, 1, 1, 1) == 25930
     assert L.intc_set_boys_mode(7) != 0
     assert b"boys mode" in L.intc_last_error()
+
+
+def test_onee_plan_argument_checks():
+    """intc_onee_plan_create refuses bad arguments before any device call
+    (runs without a GPU): unknown kind, l beyond the built kernels, shell and
+    centre indices out of range, negative sizes."""
+    import ctypes
+    import numpy as np
+    if not os.path.exists(lib.LIB_PATH):
+        import pytest
+        pytest.skip("libintc.so not built")
+    L = lib.load()
+    ip = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+    h = ctypes.c_void_p()
+    sl = np.array([0, 1, 4], dtype=np.int32)
+    snp = np.array([1, 2, 1], dtype=np.int32)
+    soff = np.array([0, 1, 3], dtype=np.int32)
+    oo = np.zeros(1, dtype=np.int64)
+    llp = oo.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong))
+
+    def create(kind, nshell, items, ncentre=1):
+        it = np.ascontiguousarray(items, dtype=np.int32)
+        return L.intc_onee_plan_create(ctypes.byref(h), kind, 1, nshell, ip(sl), ip(snp), ip(soff), None, None, None,
+                                       ncentre, 1, ip(it), llp)
+
+    assert create(2, 2, [0, 1]) == -1
+    assert b"kind" in L.intc_last_error()
+    assert create(0, 3, [0, 2]) == -2          # shell 2 has l = 4
+    assert create(0, 2, [0, 5]) == -1          # shell index out of range
+    assert create(1, 2, [0, 1, 3], ncentre=2) == -1   # centre index out of range
+    assert L.intc_onee_plan_create(ctypes.byref(h), 0, 1, -1, ip(sl), ip(snp), ip(soff), None, None, None, 0, 0,
+                                   None, None) == -1
