@@ -138,6 +138,20 @@ int intc_plan_set_route_fp32(intc_plan *plan, int use_fp32);
 int intc_plan_set_persistent(intc_plan *plan, int enable, int tile_iters, int grid);
 /* Tiles and LDS doubles per wave of the last persistent eager execution. */
 int intc_plan_persistent_info(const intc_plan *plan, int *n_tiles, int *lds_doubles_per_wave);
+/* Primitive-pair screen (no counterpart in the reference, which contracts
+ * every primitive; contracted_general.py:1083-1416): builds after this call
+ * (execute, execute_routed, graph capture, build_pairs) keep only the
+ * primitive pairs whose s-type Schwarz factor w = |c_a c_b| K_AB
+ * sqrt(2 pi^(5/2)) / zeta * (2 zeta)^(-1/4) is >= cut; cut = 0 (the default)
+ * keeps every pair and leaves results unchanged.  An integral then moves by
+ * at most about cut * (sum of w over the other pair) times an angular factor
+ * (measured <= 7 for l <= 2).  Graphs captured before the call are unchanged;
+ * intc_plan_run_bin needs a build after it.  -1 for a negative or non-finite
+ * cut. */
+int intc_plan_set_prim_screen(intc_plan *plan, double cut);
+/* Primitive pairs per shell pair in plan pair order: all (screened = 0) or
+ * those kept by the last screened build (screened = 1). */
+int intc_plan_pair_prims(const intc_plan *plan, int screened, int *out);
 /* FP64/FP32/skip queue lengths per bin of the last routed execution (3 ints per bin). */
 int intc_plan_route_counts(intc_plan *plan, int *counts);
 /* Plan order: order[i] = caller's index of the quartet at plan position i

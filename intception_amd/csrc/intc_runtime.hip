@@ -148,6 +148,41 @@ __global__ void __launch_bounds__(256) intc_build_pairs(
     }
 }
 
+// Primitive-pair screen: one thread per shell pair keeps, in order, the
+// primitive pairs whose s-type Schwarz factor w = |kappa| (2 zeta)^(-1/4)
+// reaches `cut` and moves them to the front of the pair's range (in place: the
+// write position never passes the read position); the kept count goes to
+// np_out.  |(ij|kl)| <= w_ij w_kl for s shells (1/sqrt(zeta + eta) <=
+// (4 zeta eta)^(-1/4)), so dropping primitive pairs with w < cut changes an
+// integral by at most about cut * sum(w) of the other pair times an angular
+// factor.
+__global__ void __launch_bounds__(256) intc_screen_pairs(int n_pair, const int *__restrict__ pair_off,
+                                                          const int *__restrict__ pair_np, double cut,
+                                                          double *__restrict__ pz, double *__restrict__ po2z,
+                                                          double *__restrict__ ppx, double *__restrict__ ppy,
+                                                          double *__restrict__ ppz, double *__restrict__ pkap,
+                                                          int *__restrict__ np_out)
+{
+    const int pr = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pr >= n_pair) return;
+    const int o = pair_off[pr], n = pair_np[pr];
+    int kept = 0;
+    for (int k = 0; k < n; ++k) {
+        const double z = pz[o + k], kap = pkap[o + k];
+        if (!(fabs(kap) >= cut * sqrt(sqrt(2.0 * z)))) continue;
+        if (kept != k) {
+            pz[o + kept] = z;
+            po2z[o + kept] = po2z[o + k];
+            ppx[o + kept] = ppx[o + k];
+            ppy[o + kept] = ppy[o + k];
+            ppz[o + kept] = ppz[o + k];
+            pkap[o + kept] = kap;
+        }
+        ++kept;
+    }
+    np_out[pr] = kept;
+}
+
 template <class T>
 int dev_upload(T **d, const std::vector<T> &h)
 {
@@ -198,6 +233,11 @@ struct intc_plan {
     double *pz = nullptr, *po2z = nullptr, *ppx = nullptr, *ppy = nullptr, *ppz = nullptr, *pkap = nullptr;
     double *pairA = nullptr, *pairAB = nullptr;
     bool pairs_built = false;   // primitive-pair arrays written at least once
+    // primitive-pair screen (intc_plan_set_prim_screen): cut on the s-type
+    // Schwarz factor of each primitive pair, 0 = keep every pair; the kept
+    // count of each shell pair after the last build
+    double prim_cut = 0.0;
+    int *pair_np_scr = nullptr;
     bool route_fp32 = true;     // routed runs launch the FP32-recurrence queues
     // persistent multi-class launch (intc_plan_set_persistent): the bins whose
     // class the multi kernel holds run as tiles of one launch; one buffer set
@@ -237,7 +277,7 @@ void plan_free(intc_plan *p)
                     p->pair_sb, p->pair_off, p->pair_np, p->pair_nk, p->pp_pair, p->shell_off, p->idx64,
                     p->idx32, p->idxs, p->counts, p->tile_q0, p->tile_n, p->tile_bin, p->bin_tile0,
                     p->tile_cnt, p->tile_base, p->bin_q0, p->pz, p->po2z, p->ppx, p->ppy, p->ppz, p->pkap,
-                    p->pairA, p->pairAB};
+                    p->pairA, p->pairAB, p->pair_np_scr};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
     for (intc_plan::MultiBufs *m : {&p->m_eager, &p->m_graph}) {
@@ -889,7 +929,7 @@ static int fill_args(const intc_plan *p, const double *sph, double *out, IntcEri
     memset(a, 0, sizeof(*a));
     a->boys_mode = g_boys_mode;
     a->pair_off = p->pair_off;
-    a->pair_np = p->pair_np;
+    a->pair_np = p->prim_cut > 0.0 ? p->pair_np_scr : p->pair_np;
     a->pair_nk = p->pair_nk;
     a->pair_A = p->pairA;
     a->pair_AB = p->pairAB;
@@ -920,7 +960,39 @@ INTC_API int intc_plan_build_pairs(intc_plan *p, const double *exps, const doubl
                        p->pair_sb, p->pair_off, p->pair_nk, p->shell_off, shell_cen, exps, coefs, p->pz, p->po2z,
                        p->ppx, p->ppy, p->ppz, p->pkap, p->pairA, p->pairAB);
     HIPCHK(hipGetLastError());
+    if (p->prim_cut > 0.0) {
+        hipLaunchKernelGGL(intc_screen_pairs, dim3((p->n_pair + 255) / 256), dim3(256), 0, st, p->n_pair,
+                           p->pair_off, p->pair_np, p->prim_cut, p->pz, p->po2z, p->ppx, p->ppy, p->ppz, p->pkap,
+                           p->pair_np_scr);
+        HIPCHK(hipGetLastError());
+    }
     p->pairs_built = true;
+    return 0;
+}
+
+// Primitive-pair screen for the builds (execute, routed execute, graph
+// capture, build_pairs) after this call; cut = 0 keeps every pair.  The pair
+// data must be built again before intc_plan_run_bin.
+INTC_API int intc_plan_set_prim_screen(intc_plan *p, double cut)
+{
+    if (!(cut >= 0.0) || !std::isfinite(cut)) return fail(-1, "prim_cut must be finite and >= 0");
+    if (cut > 0.0 && !p->pair_np_scr && p->n_pair > 0) {
+        if (hipMalloc(&p->pair_np_scr, sizeof(int) * p->n_pair) != hipSuccess)
+            return fail(-3, "hipMalloc screened pair counts");
+    }
+    p->prim_cut = cut;
+    p->pairs_built = false;
+    return 0;
+}
+
+// Primitive pairs of every shell pair (plan pair order): all of them
+// (screened = 0) or those the last build kept (screened = 1).
+INTC_API int intc_plan_pair_prims(const intc_plan *p, int screened, int *out)
+{
+    if (p->n_pair == 0) return 0;
+    if (screened && (p->prim_cut <= 0.0 || !p->pairs_built))
+        return fail(-1, "no screened build: set a primitive cut and build the pair data first");
+    HIPCHK(hipMemcpy(out, screened ? p->pair_np_scr : p->pair_np, sizeof(int) * p->n_pair, hipMemcpyDeviceToHost));
     return 0;
 }
 

@@ -153,6 +153,31 @@ class EriPlan:
         _lib.check(self._L.intc_plan_persistent_info(self.handle, ctypes.byref(n), ctypes.byref(r)))
         return n.value, r.value
 
+    def set_prim_screen(self, cut):
+        """Keep only primitive pairs with s-type Schwarz factor w >= cut in
+        builds after this call (0 keeps every pair); see prim_screen_cut."""
+        _lib.check(self._L.intc_plan_set_prim_screen(self.handle, float(cut)))
+
+    def pair_prims(self, screened=False):
+        """Primitive pairs per shell pair (plan pair order): all, or those
+        the last screened build kept."""
+        n = int(self._n_pair())
+        out = np.zeros(max(n, 1), dtype=np.int32)
+        _lib.check(self._L.intc_plan_pair_prims(self.handle, int(bool(screened)),
+                                                out.ctypes.data_as(ctypes.POINTER(ctypes.c_int))))
+        return out[:n]
+
+    def _n_pair(self):
+        # a plan over shared pairs keeps the caller's pairs; otherwise every
+        # quartet has its own bra and ket pair
+        b = self.batch
+        return len(b.pair_shells) if b.pair_shells is not None else 2 * b.n_quartet
+
+    def prim_screen_cut(self, eps):
+        """Primitive cut for an integral change of about eps: eps divided by
+        the largest sum of w over one shell pair of this batch."""
+        return float(eps) / max(pair_w_sums(self.batch).max(initial=0.0), 1e-300)
+
     def route_counts(self):
         n = self._L.intc_plan_num_bins(self.handle)
         buf = (ctypes.c_int * (3 * max(n, 1)))()
@@ -259,6 +284,33 @@ class OneEPlan:
             self.close()
         except Exception:
             pass
+
+
+def pair_w_sums(batch):
+    """Sum over primitive pairs of the s-type Schwarz factor
+    w = |c_a c_b| K_AB sqrt(2 pi^(5/2)) / zeta (2 zeta)^(-1/4) for every shell
+    pair of the batch (its shared pairs, else the bra and ket of each quartet),
+    the factor the device primitive screen compares with its cut."""
+    b = batch
+    if b.pair_shells is not None:
+        pairs = np.asarray(b.pair_shells, dtype=np.int64).reshape(-1, 2)
+    else:
+        q = np.asarray(b.quartets, dtype=np.int64).reshape(-1, 4)
+        pairs = np.unique(np.sort(np.concatenate([q[:, :2], q[:, 2:]]), axis=1), axis=0)
+    sa, sb = pairs[:, 0], pairs[:, 1]
+    cen = np.asarray(b.shell_cen, dtype=np.float64).reshape(-1, 3)
+    r2 = np.sum((cen[sa] - cen[sb]) ** 2, axis=1)
+    na, nb = np.asarray(b.shell_nprim)[sa], np.asarray(b.shell_nprim)[sb]
+    oa, ob = np.asarray(b.shell_off)[sa], np.asarray(b.shell_off)[sb]
+    tot = np.zeros(len(pairs))
+    for i in range(int(na.max(initial=0))):
+        for j in range(int(nb.max(initial=0))):
+            m = (i < na) & (j < nb)
+            al, be = b.exps[oa[m] + i], b.exps[ob[m] + j]
+            z = al + be
+            kap = 5.914967172795612878 * np.exp(-al * be / z * r2[m]) * b.coefs[oa[m] + i] * b.coefs[ob[m] + j] / z
+            tot[m] += np.abs(kap) * (2.0 * z) ** -0.25
+    return tot
 
 
 def set_boys_mode(mode):
