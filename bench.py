@@ -269,7 +269,38 @@ def run_molecule(args, three_center=False):
                   "tile_iters": 64, "grid_blocks": 512,
                   "note": "one launch of the multi-class kernel (every spherical register class) per step; "
                           "its occupancy is the worst class's (2 waves/SIMD), so it trails the per-class graph"}
-    ok = [v for v in (learned_mixed, learned_skip) if v["meets_1e-10"]]
+    # primitive-pair screen: the pair build inside each graph keeps only the
+    # primitive pairs whose s-type Schwarz factor reaches eps / (largest pair
+    # sum of that factor); plain FP64 and the learned routing on top of it
+    prim = None
+    learned_ps = None
+    if args.prim_eps > 0:
+        cut = plan.prim_screen_cut(args.prim_eps)
+        plan.set_prim_screen(cut)
+        plan.set_route_fp32(False)
+        plan.capture(outml, n_streams=args.streams)
+        for _ in range(args.warmup):
+            plan.replay()
+        ms_ps = _timed(torch, dev, stream, plan.replay, args.steps)
+        err_ps = max(float((outml[s0:s0 + (1 << 28)] - out64[s0:s0 + (1 << 28)]).abs().max().item())
+                     for s0 in range(0, batch.out_size, 1 << 28))
+        kept = plan.pair_prims(screened=True).astype(np.int64)
+        allp = plan.pair_prims(screened=False).astype(np.int64)
+        frac_pq = float(np.sum(kept[pp[:, 0]] * kept[pp[:, 1]]) / np.sum(allp[pp[:, 0]] * allp[pp[:, 1]]))
+        log("plain FP64 + primitive screen (cut %.3e): %.3f ms/step, primitive quartets kept %.3f, max abs err %.3e"
+            % (cut, ms_ps, frac_pq, err_ps))
+        learned_ps = run_learned(True)
+        plan.set_prim_screen(0.0)
+        plan.build_pairs()      # unscreened pair data again for the per-bin timing
+        prim = {"eps": args.prim_eps, "cut": cut, "pairs_kept_fraction": float(kept.sum() / allp.sum()),
+                "primitive_quartets_kept_fraction": frac_pq,
+                "plain_fp64_prim_screened": {"ms_per_step": float(ms_ps),
+                                             "integrals_per_s": float(batch.out_size / (ms_ps * 1e-3)),
+                                             "max_abs_err_vs_fp64": err_ps},
+                "learned_skip_and_fp32_prim_screened": learned_ps,
+                "rule": "keep primitive pair ij of a shell pair when |kappa_ij| (2 zeta_ij)^(-1/4) >= cut = eps / "
+                        "max over shell pairs of the sum of that factor"}
+    ok = [v for v in (learned_mixed, learned_skip, learned_ps) if v is not None and v["meets_1e-10"]]
     # the reported value comes from a learned variant within the error bar; if
     # neither is, the plain FP64 leg is the valid result and is reported
     best = min(ok, key=lambda v: v["ms_per_step"]) if ok else None
@@ -321,7 +352,8 @@ def run_molecule(args, three_center=False):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64+f32 (learned routing)" if best else "f64",
+        "dtype": ("f64+f32 (learned routing, primitive screen)" if best is not None and best is learned_ps else
+                  "f64+f32 (learned routing)" if best else "f64"),
         "data": "synthetic",
         "config": {"workload": (("config4: %d-atom molecule (random sequential packing, r = %.1f bohr, spacing "
                                  ">= 1.8), orbital s,s,p,p,d K~U{1..6}, 12 auxiliary shells/atom l<=4 K=1; all %d "
@@ -342,13 +374,17 @@ def run_molecule(args, three_center=False):
         "schwarz_only_at_tau_skip": schwarz_only,
         "speedup_vs_schwarz_only": float(ms_schwarz / msml),
         "persistent_launch": persistent,
+        "primitive_screen": prim,
         "value_from": ("learned_skip_and_fp32" if best is learned_mixed else
                        "learned_skip_fp64" if best is learned_skip else
+                       "primitive_screen.learned_skip_and_fp32_prim_screened" if best is learned_ps else
                        "plain_fp64 (no learned variant within 1e-10)"),
         "router_ms": float(ms_route),
         "router_mfma": router_mfma,
         "fp32_twin_of_dominant_kernel": fp32_twin,
         "speedup_vs_plain_fp64": float(ms64 / msml),
+        "speedup_vs_plain_fp64_prim_screened": (float(prim["plain_fp64_prim_screened"]["ms_per_step"] / msml)
+                                                if prim else None),
         "router": {k: v for k, v in rstats.items() if k != "train_mse"},
         "roofline": {"bound": "mfma", "achieved": float(achieved), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": float(achieved / FP64_PEAK_TFLOPS),
@@ -795,6 +831,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--streams", type=int, default=4)
+    ap.add_argument("--prim-eps", type=float, default=1e-11,
+                    help="primitive-pair screen legs of config3/4: cut for an integral change of ~eps (0: off)")
     ap.add_argument("--tune-out", default=None, help="also write the tuning table here")
     ap.add_argument("--check", type=int, default=3000, help="quartets checked against the oracle")
     args = ap.parse_args()
