@@ -361,20 +361,30 @@ def emit_class_source(cls, spherical, prec="d", waves_per_eu=None, batch=None):
     # software pipeline over the team's quartets: each pass loads the pair ids
     # of the next quartet and the queue entry of the one after it, so the
     # pass starts with its pair ids at hand (the same entries, under the same
-    # guards, read one pass earlier)
+    # guards, read one pass earlier).  A deeper pipeline (pair offsets a pass
+    # ahead too) measured no faster.  The multi-class kernel, which includes
+    # these bodies and holds every class at its register limit, keeps the
+    # plain loop head.
+    o("#ifdef INTC_BODIES_ONLY")
+    o("for (int qq = qq0 + tw; qq < qend; qq += qstep) {")
+    o("    const int q = a.q_idx ? a.q_idx[qq] : qq;")
+    o("    if (q < 0 || q >= a.nq) continue;  // queue entries index the bin")
+    o("    const int pb = a.q_bra[q], pk = a.q_ket[q];")
+    o("#else")
     o("int q1 = qq0 + tw < qend ? (a.q_idx ? a.q_idx[qq0 + tw] : qq0 + tw) : -1;")
     o("int pb1 = 0, pk1 = 0;")
     o("if (q1 >= 0 && q1 < a.nq) { pb1 = a.q_bra[q1]; pk1 = a.q_ket[q1]; }")
     o("int q2 = qq0 + tw + qstep < qend ? (a.q_idx ? a.q_idx[qq0 + tw + qstep] : qq0 + tw + qstep) : -1;")
     o("for (int qq = qq0 + tw; qq < qend; qq += qstep) {")
+    o("    const int q = q1, pb = pb1, pk = pk1;")
+    o("    q1 = q2;")
+    o("    if (q1 >= 0 && q1 < a.nq) { pb1 = a.q_bra[q1]; pk1 = a.q_ket[q1]; }")
+    o("    q2 = qq + 2 * qstep < qend ? (a.q_idx ? a.q_idx[qq + 2 * qstep] : qq + 2 * qstep) : -1;")
+    o("    if (q < 0 || q >= a.nq) continue;  // queue entries index the bin")
+    o("    const long long qoff = a.q_off[q];")
+    o("    const unsigned long long qs = a.q_str[q];")
+    o("#endif")
     o.ind = 2
-    o("const int q = q1, pb = pb1, pk = pk1;")
-    o("q1 = q2;")
-    o("if (q1 >= 0 && q1 < a.nq) { pb1 = a.q_bra[q1]; pk1 = a.q_ket[q1]; }")
-    o("q2 = qq + 2 * qstep < qend ? (a.q_idx ? a.q_idx[qq + 2 * qstep] : qq + 2 * qstep) : -1;")
-    o("if (q < 0 || q >= a.nq) continue;  // queue entries index the bin")
-    o("const long long qoff = a.q_off[q];")
-    o("const unsigned long long qs = a.q_str[q];")
     o("const int ob = a.pair_off[pb], Pb = a.pair_np[pb];")
     o("const int okk = a.pair_off[pk], Pk = a.pair_np[pk];")
     o("const double Ax = a.pair_A[3 * pb], Ay = a.pair_A[3 * pb + 1], Az = a.pair_A[3 * pb + 2];")
@@ -516,6 +526,10 @@ def emit_class_source(cls, spherical, prec="d", waves_per_eu=None, batch=None):
         o("{ const double r = intc_shfl(acc%d, src4); if (take) acc%d += r; }" % (cidx(f), cidx(f)))
     o.ind = 2
     o("}")
+    o("#ifdef INTC_BODIES_ONLY")
+    o("const long long qoff = a.q_off[q];")
+    o("const unsigned long long qs = a.q_str[q];")
+    o("#endif")
     o("const int sA = (int)(qs & 0xffff), sB = (int)((qs >> 16) & 0xffff);")
     o("const int sC = (int)((qs >> 32) & 0xffff), sD = (int)((qs >> 48) & 0xffff);")
     o("(void)sA; (void)sB;")
