@@ -162,3 +162,35 @@ def test_onee_plan_argument_checks():
     assert create(1, 2, [0, 1, 3], ncentre=2) == -1   # centre index out of range
     assert L.intc_onee_plan_create(ctypes.byref(h), 0, 1, -1, ip(sl), ip(snp), ip(soff), None, None, None, 0, 0,
                                    None, None) == -1
+
+
+def test_pair_w_sums_matches_direct_loop():
+    """runtime.pair_w_sums: the per-pair sum of the s-type Schwarz factor the
+    device primitive screen cuts on, for shared-pair and per-quartet batches."""
+    import math
+    import numpy as np
+    from intception_amd import basis, molecule, runtime
+    m = molecule.Molecule(n_atoms=6, seed=3)
+    m.pairs = m.prescreen_pairs()
+    b = m.pair_batch(np.array([[0, 1], [2, 2], [1, 3]]))
+
+    def direct(a, c):
+        tot = 0.0
+        for i in range(m.shell_nprim[a]):
+            for j in range(m.shell_nprim[c]):
+                al, be = m.exps[m.shell_off[a] + i], m.exps[m.shell_off[c] + j]
+                z = al + be
+                r2 = float(np.sum((m.shell_cen[a] - m.shell_cen[c]) ** 2))
+                kap = math.sqrt(2 * math.pi ** 2.5) * math.exp(-al * be / z * r2) / z
+                kap *= m.coefs[m.shell_off[a] + i] * m.coefs[m.shell_off[c] + j]
+                tot += abs(kap) * (2 * z) ** -0.25
+        return tot
+
+    got = runtime.pair_w_sums(b)
+    ref = np.array([direct(a, c) for a, c in m.pairs])
+    assert np.allclose(got, ref, rtol=1e-13, atol=0)
+    # per-quartet batch: the unique (sorted) bra and ket pairs
+    q = np.array([[0, 1, 2, 3], [3, 2, 1, 0], [4, 4, 0, 1]], dtype=np.int32)
+    bq = m.batch(q)
+    pairs = sorted({(0, 1), (2, 3), (4, 4)})
+    assert np.allclose(runtime.pair_w_sums(bq), [direct(a, c) for a, c in pairs], rtol=1e-13, atol=0)
