@@ -262,11 +262,36 @@ def run_molecule(args, three_center=False):
     log("plain FP64, one persistent launch: %.3f ms/step (output identical to the per-class graph: %s)"
         % (ms_p64, same))
     learned_p = run_learned(True, persistent=True)
+    # hybrid: the persistent launch holds only the cheapest bins (by the
+    # reference-scheme FLOP model, up to a share of the step's work); the heavy
+    # bins keep their class kernels on the side streams
+    flops_q = flops_model.batch_flops(batch)
+    bins_now = plan.bins()
+    q0s = np.concatenate([[0], np.cumsum([bb["nq"] for bb in bins_now])])
+    bin_cost = np.array([flops_q[q0s[i]:q0s[i + 1]].sum() for i in range(len(bins_now))])
+    hybrid = []
+    for share in (0.1, 0.25):
+        order_c = np.argsort(bin_cost)
+        cover = np.zeros(len(bins_now), dtype=bool)
+        cover[order_c[np.cumsum(bin_cost[order_c]) <= share * bin_cost.sum()]] = True
+        plan.set_persistent_bins(cover)
+        plan.capture(outml, n_streams=args.streams)
+        for _ in range(args.warmup):
+            plan.replay()
+        ms_h = _timed(torch, dev, stream, plan.replay, args.steps)
+        same_h = bool(torch.equal(outml, out64))
+        lh = run_learned(True, persistent=True)
+        log("hybrid persistent (%d cheapest bins, %.0f %% of the work): plain %.3f ms (identical: %s), learned %.3f ms"
+            % (cover.sum(), 100 * share, ms_h, same_h, lh["ms_per_step"]))
+        hybrid.append({"work_share": share, "bins_in_launch": int(cover.sum()), "plain_fp64_ms_per_step": float(ms_h),
+                       "plain_output_identical": same_h, "learned_skip_and_fp32_ms_per_step": lh["ms_per_step"],
+                       "learned_max_abs_err_vs_fp64": lh["max_abs_err_vs_fp64"]})
+    plan.set_persistent_bins(None)
     plan.set_persistent(False)
     persistent = {"plain_fp64_ms_per_step": float(ms_p64), "plain_output_identical": same,
                   "learned_skip_and_fp32_ms_per_step": learned_p["ms_per_step"],
                   "learned_max_abs_err_vs_fp64": learned_p["max_abs_err_vs_fp64"],
-                  "tile_iters": 64, "grid_blocks": 512,
+                  "tile_iters": 64, "grid_blocks": 512, "hybrid_cheapest_bins": hybrid,
                   "note": "one launch of the multi-class kernel (every spherical register class) per step; "
                           "its occupancy is the worst class's (2 waves/SIMD), so it trails the per-class graph"}
     # primitive-pair screen: the pair build inside each graph keeps only the
@@ -305,7 +330,6 @@ def run_molecule(args, three_center=False):
     # neither is, the plain FP64 leg is the valid result and is reported
     best = min(ok, key=lambda v: v["ms_per_step"]) if ok else None
     msml = best["ms_per_step"] if best else ms64
-    flops_q = flops_model.batch_flops(batch)
     canon = canonical_np(batch.classes())
     per_bin = _per_bin(torch, plan, out64, flops_q, canon, stream)
     dom = max(per_bin, key=lambda r: r["ms"])

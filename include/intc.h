@@ -136,6 +136,13 @@ int intc_plan_set_route_fp32(intc_plan *plan, int use_fp32);
  * with the built-in transforms (sph = NULL) and Boys mode 0 only (-2 / -1
  * otherwise).  grid: blocks of 256 threads, 0 = two per CU. */
 int intc_plan_set_persistent(intc_plan *plan, int enable, int tile_iters, int grid);
+/* Bins the persistent launch may hold (no counterpart in the reference):
+ * cover[i] != 0 selects bin i (intc_plan_bin_info order); n = 0 clears the
+ * selection (every bin whose class the multi-class kernel holds).  With the
+ * persistent launch enabled, the selected bins run as tiles of the one launch
+ * and the others keep their own class kernels beside it.  -1 unless n is 0 or
+ * the number of bins. */
+int intc_plan_set_persistent_bins(intc_plan *plan, int n, const int *cover);
 /* Tiles and LDS doubles per wave of the last persistent eager execution. */
 int intc_plan_persistent_info(const intc_plan *plan, int *n_tiles, int *lds_doubles_per_wave);
 /* Primitive-pair screen (no counterpart in the reference, which contracts

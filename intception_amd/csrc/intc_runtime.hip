@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <array>
 #include <cmath>
+#include <cstddef>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -246,6 +247,7 @@ struct intc_plan {
     int tile_iters = 64;                     // iterations of each team per tile (at most)
     int multi_grid = 2 * 256;                // blocks of 256 (two per CU)
     std::vector<long long> prim_prefix;      // primitive quartets before each plan position
+    std::vector<char> multi_bin;             // bins the multi kernel may hold (empty: all)
     struct MultiBufs {
         IntcEriArgs *args = nullptr;         // [2 * bins]: bin b, precision r at 2b + r
         int4 *tiles = nullptr;
@@ -1069,7 +1071,10 @@ static int reg_doubles(const IntcClassEntry *c, int T)
 
 static bool multi_covers(const intc_plan *p, const IntcBin &b, int prec)
 {
-    return p->persistent && b.cls[prec]->multi >= 0;
+    if (!p->persistent || b.cls[prec]->multi < 0) return false;
+    if (p->multi_bin.empty()) return true;
+    const ptrdiff_t i = &b - p->bins.data();
+    return i >= 0 && i < (ptrdiff_t)p->multi_bin.size() && p->multi_bin[i];
 }
 
 // Argument array and tile list of the persistent launch.  A tile is a run of
@@ -1284,6 +1289,17 @@ INTC_API int intc_plan_set_persistent(intc_plan *p, int enable, int tile_iters, 
     p->persistent = enable != 0;
     p->tile_iters = tile_iters;
     p->multi_grid = grid ? grid : 2 * 256;
+    return 0;
+}
+
+// Bins the persistent launch may hold (cover[i] != 0 for bin i); n = 0 clears
+// the selection (every bin whose class the multi kernel holds).  The other
+// bins keep their own class kernels on the side streams.
+INTC_API int intc_plan_set_persistent_bins(intc_plan *p, int n, const int *cover)
+{
+    if (n != 0 && n != (int)p->bins.size()) return fail(-1, "cover needs one entry per bin (or n = 0)");
+    p->multi_bin.assign(n, 0);
+    for (int i = 0; i < n; ++i) p->multi_bin[i] = cover[i] != 0;
     return 0;
 }
 

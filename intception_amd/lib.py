@@ -46,6 +46,7 @@ SIGNATURES = [
     ("intc_plan_set_route_fp32", c_int, [c_void_p, c_int]),
     ("intc_plan_set_persistent", c_int, [c_void_p, c_int, c_int, c_int]),
     ("intc_plan_persistent_info", c_int, [c_void_p, _ip, _ip]),
+    ("intc_plan_set_persistent_bins", c_int, [c_void_p, c_int, _ip]),
     ("intc_plan_set_prim_screen", c_int, [c_void_p, c_double]),
     ("intc_plan_pair_prims", c_int, [c_void_p, c_int, _ip]),
     ("intc_plan_route_counts", c_int, [c_void_p, _ip]),

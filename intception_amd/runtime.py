@@ -147,6 +147,17 @@ class EriPlan:
         captures."""
         _lib.check(self._L.intc_plan_set_persistent(self.handle, int(bool(enable)), int(tile_iters), int(grid)))
 
+    def set_persistent_bins(self, cover=None):
+        """Bins the persistent launch may hold: a boolean per bin (bins()
+        order), or None for every bin its kernel holds.  The others keep their
+        own class kernels."""
+        if cover is None:
+            _lib.check(self._L.intc_plan_set_persistent_bins(self.handle, 0, None))
+            return
+        c = np.ascontiguousarray(np.asarray(cover, dtype=bool).astype(np.int32))
+        _lib.check(self._L.intc_plan_set_persistent_bins(self.handle, len(c),
+                                                         c.ctypes.data_as(ctypes.POINTER(ctypes.c_int))))
+
     def persistent_info(self):
         """(tiles, LDS doubles per wave) of the last persistent eager run."""
         n, r = ctypes.c_int(), ctypes.c_int()

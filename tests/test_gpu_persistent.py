@@ -127,3 +127,39 @@ def test_persistent_rejections(gpu):
     finally:
         runtime.set_boys_mode(0)
     plan.close()
+
+
+def test_persistent_subset_of_bins(gpu):
+    """intc_plan_set_persistent_bins: the persistent launch holds only the
+    selected bins, the others run their own class kernels beside it; plain,
+    routed and captured runs give the per-class output bit for bit."""
+    import torch
+    b = basis.config1(n_quartet=6000, seed=13)
+    print("persistent subset: config-1 classes, 6000 quartets, seed 13", flush=True)
+    plan = runtime.EriPlan(b)
+    ref = plan.run().cpu().numpy()
+    nb = len(plan.bins())
+    rng = np.random.default_rng(3)
+    cover = rng.random(nb) < 0.5
+    plan.set_persistent(True, 8)
+    plan.set_persistent_bins(cover)
+    got = plan.run().cpu().numpy()
+    assert np.array_equal(got, ref)
+    route = torch.from_numpy(rng.integers(0, 3, size=b.n_quartet).astype(np.int8)).to(plan.device)
+    plan.set_persistent(False)
+    rref = plan.run_routed(route).cpu().numpy()
+    plan.set_persistent(True, 8)
+    rgot = plan.run_routed(route).cpu().numpy()
+    assert np.array_equal(rgot, rref)
+    out = plan.alloc_out()
+    plan.capture(out, n_streams=4, route=route)
+    for rep in range(3):
+        out.fill_(float("nan"))
+        plan.replay()
+        torch.cuda.synchronize()
+        assert np.array_equal(out.cpu().numpy(), rref), "graph replay %d" % rep
+    plan.set_persistent_bins(None)
+    assert np.array_equal(plan.run().cpu().numpy(), ref)
+    with pytest.raises(Exception):
+        plan.set_persistent_bins(np.ones(nb + 1, dtype=bool))
+    plan.close()
