@@ -887,17 +887,16 @@ def emit_onee_source(lmax=ONEE_LMAX):
 
 def multi_members(metas):
     """Class kernels compiled into the persistent multi-class kernel: every
-    spherical register kernel, FP64 and FP32 recurrences, except the FP32
-    kernels of classes with an f or g shell (switched off in the runtime,
-    which gives those bins the FP64 kernel)."""
-    out = []
-    for m in metas:
-        if not m["spherical"]:
-            continue
-        if m["prec"] == "f" and max(m["cls"]) >= 3:
-            continue
-        out.append(m)
-    return out
+    spherical register kernel, FP64 and FP32 recurrences."""
+    return [m for m in metas if m["spherical"]]
+
+
+def fp32_register_class(cls):
+    """Whether a register class gets an FP32-recurrence kernel: shells up to
+    l = 2 only.  The three-centre classes with an f or g auxiliary shell have
+    none (one routed run of those kernels stopped responding and no cause was
+    found); the runtime gives their FP32-routed quartets the FP64 kernel."""
+    return max(cls) <= 2
 
 
 def emit_multi_source(members):
@@ -973,6 +972,8 @@ def generate(outdir, classes, sph_modes=(False, True), precs=("d", "f"), wg=(), 
     written = []
     for cls in classes:
         for sph, prec in itertools.product(sph_modes, precs):
+            if prec == "f" and not fp32_register_class(cls):
+                continue
             nm = kernel_name(cls, sph, prec)
             src, meta = emit_class_source(cls, sph, prec, (occupancy or {}).get(nm), (batch or {}).get(nm))
             fn = os.path.join(outdir, meta["name"] + ".hip")

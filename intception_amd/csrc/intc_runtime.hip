@@ -564,13 +564,11 @@ static int finish_plan(intc_plan *p, const std::vector<const IntcClassEntry *> &
         while (j < nq && qcls[j] == c) sum += prims[j++];
         IntcBin b;
         b.cls[0] = c;
+        // FP32-recurrence kernel; register classes with an f or g shell (the
+        // three-centre (ab|P s) classes) have none, and their FP32-routed
+        // quartets run the FP64 kernel
         b.cls[1] = find_class(c->la, c->lb, c->lc, c->ld, p->spherical, 1);
-        // The FP32-recurrence register kernels of classes with an f or g shell
-        // (the three-centre (ab|P s) classes) are switched off: their first
-        // routed run stopped responding and the cause is not known.  Quartets
-        // routed to FP32 in these classes run the FP64 kernel.
-        const bool high_l = std::max(std::max(c->la, c->lb), std::max(c->lc, c->ld)) >= 3;
-        if (!b.cls[1] || (high_l && !c->wg)) b.cls[1] = c;
+        if (!b.cls[1]) b.cls[1] = c;
         b.q0 = (int)i;
         b.nq = (int)(j - i);
         b.block = 256;

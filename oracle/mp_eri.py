@@ -189,3 +189,71 @@ def contracted_eri(shells, spherical=False, sph=None):
                                     v += mp.mpf(wa) * wb * wc * wd * out[(ka, kb, kc, kd)]
                     res.append(v)
     return res
+
+
+def contracted_eri_elements(shells, elements, spherical=False, sph=None):
+    """Selected elements (flat row-major [a][b][c][d] indices into the block)
+    of a contracted quartet, same definition as contracted_eri but evaluated
+    per primitive quartet with one Hermite R table and one set of E tables
+    shared by every Cartesian component the elements need (for blocks too
+    large to evaluate whole, e.g. (ff|ff) with several primitives)."""
+    comps = [cart_components(s["l"]) for s in shells]
+    ncomp = [(2 * s["l"] + 1) if spherical else len(c) for s, c in zip(shells, comps)]
+    # Cartesian terms each element needs: (weight, (ia, ib, ic, id))
+    need = {}
+    plans = []
+    for el in elements:
+        idx = []
+        r = int(el)
+        for n in reversed(ncomp):
+            idx.append(r % n)
+            r //= n
+        idx = idx[::-1]
+        terms = []
+        if spherical:
+            lists = []
+            for k in range(4):
+                col = [(ic, sph[shells[k]["l"]][ic][idx[k]]) for ic in range(len(comps[k]))]
+                lists.append([(ic, w) for ic, w in col if w != 0])
+            for (ia, wa), (ib, wb), (ic, wc), (idd, wd) in itertools.product(*lists):
+                terms.append((mp.mpf(wa) * wb * wc * wd, (ia, ib, ic, idd)))
+        else:
+            terms.append((mp.mpf(1), tuple(idx)))
+        for _, key in terms:
+            need[key] = mp.mpf(0)
+        plans.append(terms)
+    cen = [[mp.mpf(v) for v in s["cen"]] for s in shells]
+    A, B, C, D = cen
+    lsum = sum(s["l"] for s in shells)
+    for (ea, wa), (eb, wb), (ec, wc), (ed, wd) in itertools.product(
+            *[list(zip(s["exps"], s["coefs"])) for s in shells]):
+        a, b, c, d = (mp.mpf(float(v)) for v in (ea, eb, ec, ed))
+        p, q = a + b, c + d
+        P = [(a * A[k] + b * B[k]) / p for k in range(3)]
+        Q = [(c * C[k] + d * D[k]) / q for k in range(3)]
+        alpha = p * q / (p + q)
+        R = _hermite_R(lsum, alpha, [P[k] - Q[k] for k in range(3)])
+        Eab = {}
+        Ecd = {}
+        w = mp.mpf(float(wa)) * mp.mpf(float(wb)) * mp.mpf(float(wc)) * mp.mpf(float(wd))
+        pref = 2 * mp.pi ** mp.mpf(2.5) / (p * q * mp.sqrt(p + q)) * w
+        for key in need:
+            ka, kb, kc, kd = (comps[k][key[k]] for k in range(4))
+            e1s = []
+            for k in range(3):
+                if (k, ka[k], kb[k]) not in Eab:
+                    Eab[(k, ka[k], kb[k])] = _hermite_E(ka[k], kb[k], a, b, A[k] - B[k])
+                if (k, kc[k], kd[k]) not in Ecd:
+                    Ecd[(k, kc[k], kd[k])] = _hermite_E(kc[k], kd[k], c, d, C[k] - D[k])
+                e1s.append((Eab[(k, ka[k], kb[k])], Ecd[(k, kc[k], kd[k])]))
+            s = mp.mpf(0)
+            (Ex, Fx), (Ey, Fy), (Ez, Fz) = e1s
+            for t, u, v in itertools.product(range(len(Ex)), range(len(Ey)), range(len(Ez))):
+                e1 = Ex[t] * Ey[u] * Ez[v]
+                if e1 == 0:
+                    continue
+                for tau, nu, phi in itertools.product(range(len(Fx)), range(len(Fy)), range(len(Fz))):
+                    e2 = Fx[tau] * Fy[nu] * Fz[phi]
+                    s += e1 * e2 * (-1) ** (tau + nu + phi) * R(t + tau, u + nu, v + phi)
+            need[key] += pref * s
+    return [sum(wt * need[key] for wt, key in terms) for terms in plans]

@@ -8,11 +8,16 @@
 * the FP32-recurrence variant: max absolute and relative error against FP64
   stated and bounded;
 * routed execution over workgroup bins: FP64 / FP32 / skip codes give exactly
-  the FP64 block, the FP32 block and zeros.
+  the FP64 block, the FP32 block and zeros;
+* the two config-2 (ff|dd) quartets on which the GPU and the oracle differed
+  beyond the FP64 bar, against their 40-digit McMurchie-Davidson values
+  (tests/golden/config2_arbitration.npz): the GPU meets the bar there.
 """
+
 import numpy as np
 import pytest
 
+from arbitration import arbitration_batch
 from conftest import assert_parity
 from intception_amd import basis, codegen, runtime
 from oracle import oracle
@@ -99,3 +104,12 @@ def test_routed_workgroup_bins(gpu):
         o, n = b.out_off[q], sz[q]
         want = ref64[o:o + n] if codes[q] == 0 else (all32[o:o + n] if codes[q] == 1 else np.zeros(n))
         assert np.array_equal(got[o:o + n], want), q
+
+
+def test_config2_arbitrated_elements(gpu):
+    b, g = arbitration_batch("config2")
+    got = _gpu(b)
+    pos = b.out_off[g["elem_q"]] + g["elem"]
+    print("config2 arbitration: %d elements (%d disputed), max |gpu - mp| %.3e" % (
+        len(pos), int(g["disputed"].sum()), np.abs(got[pos] - g["val"]).max()), flush=True)
+    assert_parity(got[pos], g["val"], what="config2 (ff|dd) elements vs 40-digit MD")

@@ -186,3 +186,33 @@ def test_work_array_size_matches_reference_generated():
         assert oracle.work_array_size(sph, ls, ks, kc) == ref, r
         f = L.intc_eri_cont_sph_work_array_size if sph else L.intc_eri_cont_cart_work_array_size
         assert f(*[int(v) for v in ls], *[int(v) for v in ks], *[int(v) for v in kc]) == ref, r
+
+
+def test_oracle_on_config2_arbitrated_elements():
+    """The oracle on the config-2 (ff|dd) elements where it and the GPU
+    differed: against their 40-digit values it is off by ~1e-14 absolute
+    (about 3e-10 of the block maximum; the a->c->d scheme's rounding), which
+    is what moved them outside the 1e-14 + 1e-12 |ref| bar."""
+    from arbitration import arbitration_batch
+    b, g = arbitration_batch("config2")
+    oracle.set_boys_mode(oracle.BOYS_EXACT)
+    ref = oracle.eri_batch(b, True, basis.SPH)
+    pos = b.out_off[g["elem_q"]] + g["elem"]
+    err = np.abs(ref[pos] - g["val"])
+    assert err.max() < 5e-14
+    assert err[g["disputed"]].min() > 5e-15
+
+
+def test_oracle_on_config4_arbitrated_elements():
+    """The same for the three-centre (ab|P s) elements of the 100-atom
+    molecule where the oracle and the GPU differed: the oracle is off by up to
+    ~2.4e-14 absolute against the 40-digit values (blocks with maxima of
+    1e-7 to 2e-2), the bar test_gpu_config4 allows it."""
+    from arbitration import arbitration_batch
+    b, g = arbitration_batch("config4")
+    oracle.set_boys_mode(oracle.BOYS_EXACT)
+    ref = oracle.eri_batch(b, True, basis.SPH)
+    pos = b.out_off[g["elem_q"]] + g["elem"]
+    err = np.abs(ref[pos] - g["val"])
+    assert err.max() < 3e-14
+    assert err[g["disputed"]].min() > 5e-15
