@@ -63,6 +63,20 @@ int intc_eri_cont_sph_main(INTC_SHELL_ARGS(a), const double *sph_trans_array_a, 
                            const double *sph_trans_array_b, INTC_SHELL_ARGS(c), const double *sph_trans_array_c,
                            INTC_SHELL_ARGS(d), const double *sph_trans_array_d, double *work_array,
                            double *output_array);
+/* No counterpart in the reference.  intc_eri_cont_sph_main for callers that
+ * cannot allocate device memory (code under HIP graph capture excluded: it
+ * synchronises; tight serving loops, harnesses that forbid hipMalloc): same
+ * arguments, same kernels and results, and no hipMalloc/hipFree.  Every
+ * device buffer lives in one static 8 MiB device array per GPU whose address
+ * intc_init() looks up, so intc_init() must have run on the current device
+ * (-1 otherwise).  Limits: n_prim <= 8 and n_cont <= 2 per index, indices
+ * with the same l pass the same sph_trans_array, all contraction blocks
+ * together <= 1 033 408 doubles (-1 beyond).  Calls that overlap on one GPU
+ * share the array and race. */
+int intc_eri_cont_sph_main_noalloc(INTC_SHELL_ARGS(a), const double *sph_trans_array_a, INTC_SHELL_ARGS(b),
+                                   const double *sph_trans_array_b, INTC_SHELL_ARGS(c),
+                                   const double *sph_trans_array_c, INTC_SHELL_ARGS(d),
+                                   const double *sph_trans_array_d, double *work_array, double *output_array);
 /* Replaces <full>_main (Cartesian variant, no sph_trans_array arguments). */
 int intc_eri_cont_cart_main(INTC_SHELL_ARGS(a), INTC_SHELL_ARGS(b), INTC_SHELL_ARGS(c), INTC_SHELL_ARGS(d),
                             double *work_array, double *output_array);

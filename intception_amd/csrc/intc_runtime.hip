@@ -516,12 +516,51 @@ int intc_internal_tables(const double **cheb, const double **small, const double
 
 int intc_internal_boys_mode() { return g_boys_mode; }
 
+// Scratch of the allocation-free per-quartet twin (intc_eri_cont_sph_main_noalloc):
+// one static device array per GPU, its device address looked up once per
+// device by intc_init.  Layout: NoAllocHead (inputs and index tables, copied
+// from the host), the primitive-pair arrays, then the output blocks.
+namespace {
+constexpr int NA_PRIM = 8;                        // primitives per shell
+constexpr int NA_CONT = 2;                        // contractions per shell
+constexpr int NA_SHELL = 4 * NA_CONT;
+constexpr int NA_COMBO = NA_CONT * NA_CONT * NA_CONT * NA_CONT;
+constexpr int NA_PAIR = 2 * NA_COMBO;             // one bra and one ket pair per combination
+constexpr int NA_PP = NA_PAIR * NA_PRIM * NA_PRIM;
+constexpr size_t NA_SCRATCH_DOUBLES = (size_t)1 << 20;   // 8 MiB per device
+constexpr int NA_MAX_DEV = 64;
+struct NoAllocHead {
+    double exps[NA_SHELL * NA_PRIM], coefs[NA_SHELL * NA_PRIM], cen[NA_SHELL * 3];
+    double sph[intc_sph_off(8)];
+    long long q_off[NA_COMBO];
+    unsigned long long q_str[NA_COMBO];
+    int shell_off[NA_SHELL];
+    int pair_sa[NA_PAIR], pair_sb[NA_PAIR], pair_off[NA_PAIR], pair_np[NA_PAIR], pair_nk[NA_PAIR];
+    int pp_pair[NA_PP];
+    int q_bra[NA_COMBO], q_ket[NA_COMBO];
+};
+constexpr size_t NA_HEAD_DOUBLES = (sizeof(NoAllocHead) + 255) / 256 * 256 / sizeof(double);
+constexpr size_t NA_OUT_OFFSET = NA_HEAD_DOUBLES + 6 * (size_t)NA_PP + 6 * (size_t)NA_PAIR;
+constexpr size_t NA_OUT_DOUBLES = NA_SCRATCH_DOUBLES - NA_OUT_OFFSET;
+double *g_scratch_ptr[NA_MAX_DEV] = {};
+}  // namespace
+static __device__ __align__(256) double scratch[NA_SCRATCH_DOUBLES];
+
 INTC_API int intc_init(void)
 {
     int dev = 0;
     HIPCHK(hipGetDevice(&dev));
     DevTables *t;
-    return ensure_tables(dev, &t);
+    int rc = ensure_tables(dev, &t);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (dev < NA_MAX_DEV && !g_scratch_ptr[dev]) {
+        // the host-side name `scratch` is a stand-in; kernels get this address
+        double *ptr = nullptr;
+        HIPCHK(hipGetSymbolAddress((void **)&ptr, HIP_SYMBOL(scratch)));
+        g_scratch_ptr[dev] = ptr;
+    }
+    return 0;
 }
 
 INTC_API int intc_class_supported(int la, int lb, int lc, int ld, int spherical)
@@ -1530,6 +1569,12 @@ static int per_quartet(int spherical, const ShellArg sa[4], double *output)
         sph_all.assign(intc_sph_off(8), 0.0);
         for (int k = 0; k < 4; ++k) {
             int l = sa[k].l;
+            // one S_l per l reaches the kernels
+            for (int k2 = 0; k2 < k; ++k2)
+                if (sa[k2].l == l && memcmp(sa[k2].sph, sa[k].sph, sizeof(double) * ncc(l) * nsph(l)) != 0) {
+                    intc_plan_destroy(p);
+                    return fail(-1, "indices with the same l must pass the same sph_trans_array");
+                }
             for (int i = 0; i < ncc(l) * nsph(l); ++i) sph_all[intc_sph_off(l) + i] = sa[k].sph[i];
         }
     }
@@ -1581,6 +1626,193 @@ static int per_quartet(int spherical, const ShellArg sa[4], double *output)
     return 0;
 }
 
+// Allocation-free variant of per_quartet for spherical output: the same
+// canonical class, pair orientation, team size and kernels, with every device
+// buffer in this device's `scratch` (intc_init must have run on the device).
+// Host memory: a NoAllocHead and a chunk buffer on the stack.  Limits: n_prim
+// <= NA_PRIM, n_cont <= NA_CONT per index, all blocks <= NA_OUT_DOUBLES doubles.
+// Calls that overlap on one GPU share the array and race.
+static int per_quartet_noalloc(const ShellArg sa[4], double *output)
+{
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    double *scr = nullptr;
+    const DevTables *tab = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        if (dev < NA_MAX_DEV) scr = g_scratch_ptr[dev];
+        if (dev < (int)g_tables.size() && g_tables[dev].cheb) tab = &g_tables[dev];
+    }
+    if (!scr || !tab) return fail(-1, "intc_init() has not run on this device");
+    int nk = 1, nblk = 1, ls[4];
+    for (int k = 0; k < 4; ++k) {
+        const ShellArg &x = sa[k];
+        if (x.l < 0 || x.l > 7 || x.n_prim < 1 || x.n_cont < 1) return fail(-1, "invalid shell arguments");
+        if (x.n_prim > NA_PRIM || x.n_cont > NA_CONT)
+            return fail(-1, "noalloc entry point: at most 8 primitives and 2 contractions per shell");
+        if (x.n != nsph(x.l)) return fail(-1, "component count n does not match l");
+        nk *= x.n_cont;
+        nblk *= x.n;
+        ls[k] = x.l;
+    }
+    if ((size_t)nk * nblk > NA_OUT_DOUBLES) return fail(-1, "noalloc entry point: output blocks exceed the scratch size");
+    int cls[4], perm0[4];
+    canonical(ls, cls, perm0);
+    const IntcClassEntry *c = find_class(cls[0], cls[1], cls[2], cls[3], 1, 0);
+    if (!c) return class_error(ls[0], ls[1], ls[2], ls[3], 1);
+    NoAllocHead h;
+    memset(&h, 0, sizeof(h));
+    // shells: one per (index, contraction) with that contraction's column
+    int base[4], ns = 0, np = 0;
+    for (int k = 0; k < 4; ++k) {
+        base[k] = ns;
+        for (int j = 0; j < sa[k].n_cont; ++j, ++ns) {
+            h.shell_off[ns] = np;
+            for (int i = 0; i < 3; ++i) h.cen[3 * ns + i] = sa[k].c[i];
+            for (int i = 0; i < sa[k].n_prim; ++i, ++np) {
+                h.exps[np] = sa[k].x[i];
+                h.coefs[np] = sa[k].contract[i + j * sa[k].n_prim];
+            }
+        }
+    }
+    // the caller's S_l, one per l: indices with the same l must pass the same matrix
+    for (int k = 0; k < 4; ++k) {
+        const int l = sa[k].l, n = ncc(l) * nsph(l);
+        for (int k2 = 0; k2 < k; ++k2)
+            if (sa[k2].l == l && memcmp(sa[k2].sph, sa[k].sph, sizeof(double) * n) != 0)
+                return fail(-1, "indices with the same l must pass the same sph_trans_array");
+        memcpy(h.sph + intc_sph_off(l), sa[k].sph, sizeof(double) * n);
+    }
+    int snp[NA_SHELL];
+    for (int k = 0; k < 4; ++k)
+        for (int j = 0; j < sa[k].n_cont; ++j) snp[base[k] + j] = sa[k].n_prim;
+    const int st_dense[4] = {sa[1].n * sa[2].n * sa[3].n, sa[2].n * sa[3].n, sa[3].n, 1};
+    int nq = 0, npair = 0, npp = 0, prims = 0;
+    for (int ka = 0; ka < sa[0].n_cont; ++ka)
+        for (int kb = 0; kb < sa[1].n_cont; ++kb)
+            for (int kc = 0; kc < sa[2].n_cont; ++kc)
+                for (int kd = 0; kd < sa[3].n_cont; ++kd, ++nq) {
+                    const int qs[4] = {base[0] + ka, base[1] + kb, base[2] + kc, base[3] + kd};
+                    int perm[4] = {perm0[0], perm0[1], perm0[2], perm0[3]};
+                    for (int hh = 0; hh < 4; hh += 2)
+                        if (cls[hh] == cls[hh + 1] &&
+                            vrr_on_second(qs[perm[hh]], qs[perm[hh + 1]], snp, h.shell_off, h.cen, h.exps, h.coefs))
+                            std::swap(perm[hh], perm[hh + 1]);
+                    for (int side = 0; side < 2; ++side, ++npair) {
+                        const int s0 = qs[perm[2 * side]], s1 = qs[perm[2 * side + 1]];
+                        h.pair_sa[npair] = s0;
+                        h.pair_sb[npair] = s1;
+                        h.pair_off[npair] = npp;
+                        h.pair_np[npair] = snp[s0] * snp[s1];
+                        h.pair_nk[npair] = snp[s1];
+                        for (int t = 0; t < snp[s0] * snp[s1]; ++t) h.pp_pair[npp++] = npair;
+                    }
+                    h.q_bra[nq] = npair - 2;
+                    h.q_ket[nq] = npair - 1;
+                    h.q_off[nq] = (long long)nq * nblk;
+                    unsigned long long pk = 0;
+                    for (int k = 0; k < 4; ++k) pk |= (unsigned long long)(st_dense[perm[k]] & 0xffff) << (16 * k);
+                    h.q_str[nq] = pk;
+                    prims += snp[qs[0]] * snp[qs[1]] * snp[qs[2]] * snp[qs[3]];
+                }
+    // device view of the scratch layout
+    const size_t o_head = 0, o_pp = NA_HEAD_DOUBLES;
+    NoAllocHead *dh = (NoAllocHead *)(scr + o_head);
+    double *pz = scr + o_pp, *po2z = pz + NA_PP, *ppx = po2z + NA_PP, *ppy = ppx + NA_PP, *ppz = ppy + NA_PP,
+           *pkap = ppz + NA_PP, *pairA = pkap + NA_PP, *pairAB = pairA + 3 * NA_PAIR;
+    double *dout = scr + NA_OUT_OFFSET;
+    HIPCHK(hipMemcpy(dh, &h, sizeof(h), hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(intc_build_pairs, dim3((npp + 255) / 256), dim3(256), 0, (hipStream_t)0, (long long)npp,
+                       dh->pp_pair, dh->pair_sa, dh->pair_sb, dh->pair_off, dh->pair_nk, dh->shell_off, dh->cen,
+                       dh->exps, dh->coefs, pz, po2z, ppx, ppy, ppz, pkap, pairA, pairAB);
+    HIPCHK(hipGetLastError());
+    IntcEriArgs a;
+    memset(&a, 0, sizeof(a));
+    a.boys_mode = g_boys_mode;
+    a.pair_off = dh->pair_off;
+    a.pair_np = dh->pair_np;
+    a.pair_nk = dh->pair_nk;
+    a.pair_A = pairA;
+    a.pair_AB = pairAB;
+    a.pp_zeta = pz;
+    a.pp_o2z = po2z;
+    a.pp_px = ppx;
+    a.pp_py = ppy;
+    a.pp_pz = ppz;
+    a.pp_kappa = pkap;
+    a.sph = dh->sph;
+    a.sph_builtin = 0;
+    a.boys_cheb = tab->cheb;
+    a.boys_small = tab->small;
+    a.boys_medium = tab->medium;
+    a.out = dout;
+    a.q_bra = dh->q_bra;
+    a.q_ket = dh->q_ket;
+    a.q_off = dh->q_off;
+    a.q_str = dh->q_str;
+    a.nq = nq;
+    // team size and grid as a plan bin of these quartets (finish_plan, grid_for)
+    IntcBin b;
+    b.cls[0] = b.cls[1] = c;
+    b.q0 = 0;
+    b.nq = nq;
+    b.block = 256;
+    b.team_groups[0] = b.team_groups[1] = default_team_groups(c, nq, (double)prims / nq);
+    b.grid[0] = b.grid[1] = grid_for(b, 0);
+    a.team_groups = b.team_groups[0];
+    a.wg_cls[0] = c->la;
+    a.wg_cls[1] = c->lb;
+    a.wg_cls[2] = c->lc;
+    a.wg_cls[3] = c->ld;
+    a.wg_cls[4] = c->sph;
+    const int tpw = 64 / (c->G * a.team_groups);
+    const size_t lds = c->wg ? 0 : (size_t)(b.block / 64) * tpw * c->NE * c->NCD * sizeof(double);
+    hipError_t e = c->fn(&a, b.grid[0], b.block, lds, (hipStream_t)0);
+    if (e != hipSuccess) return fail(-3, std::string("class kernel launch: ") + hipGetErrorString(e));
+    HIPCHK(hipDeviceSynchronize());
+    // results back in chunks, scattered to out[sum_X(iX*iskipX + icontX*iskipX_cont)]
+    double buf[4096];
+    const long long total = (long long)nq * nblk;
+    long long done = 0;
+    int q = 0, ia = 0, ib = 0, ic = 0, id = 0, ca = 0, cb = 0, cc = 0, cd = 0;
+    while (done < total) {
+        const int n = (int)std::min<long long>(4096, total - done);
+        HIPCHK(hipMemcpy(buf, dout + done, sizeof(double) * n, hipMemcpyDeviceToHost));
+        for (int i = 0; i < n; ++i) {
+            output[(long)ia * sa[0].iskip + (long)ca * sa[0].iskip_cont + (long)ib * sa[1].iskip +
+                   (long)cb * sa[1].iskip_cont + (long)ic * sa[2].iskip + (long)cc * sa[2].iskip_cont +
+                   (long)id * sa[3].iskip + (long)cd * sa[3].iskip_cont] = buf[i];
+            // next element: d fastest within a block, blocks in (ka, kb, kc, kd) order
+            if (++id == sa[3].n) {
+                id = 0;
+                if (++ic == sa[2].n) {
+                    ic = 0;
+                    if (++ib == sa[1].n) {
+                        ib = 0;
+                        if (++ia == sa[0].n) {
+                            ia = 0;
+                            ++q;
+                            if (++cd == sa[3].n_cont) {
+                                cd = 0;
+                                if (++cc == sa[2].n_cont) {
+                                    cc = 0;
+                                    if (++cb == sa[1].n_cont) {
+                                        cb = 0;
+                                        ++ca;
+                                    }
+                                }
+                            }
+                        }
+                    }
+                }
+            }
+        }
+        done += n;
+    }
+    (void)q;
+    return 0;
+}
+
 #define INTC_SHELL_PARAMS(X)                                                                        \
     const double *x##X, const double c##X[3], const int l##X, const int n##X, const int iskip##X,   \
         const int n##X##_prim, const int n##X##_cont, const int iskip##X##_cont,                    \
@@ -1597,6 +1829,20 @@ INTC_API int intc_eri_cont_sph_main(INTC_SHELL_PARAMS(a), const double *sph_tran
     ShellArg s[4] = {INTC_MK(a, sph_trans_array_a), INTC_MK(b, sph_trans_array_b), INTC_MK(c, sph_trans_array_c),
                      INTC_MK(d, sph_trans_array_d)};
     return per_quartet(1, s, output_array);
+}
+
+// No counterpart in the reference: intc_eri_cont_sph_main without any device
+// allocation (scratch in a static device array; limits above).
+INTC_API int intc_eri_cont_sph_main_noalloc(INTC_SHELL_PARAMS(a), const double *sph_trans_array_a,
+                                            INTC_SHELL_PARAMS(b), const double *sph_trans_array_b,
+                                            INTC_SHELL_PARAMS(c), const double *sph_trans_array_c,
+                                            INTC_SHELL_PARAMS(d), const double *sph_trans_array_d,
+                                            double *work_array, double *output_array)
+{
+    (void)work_array;
+    ShellArg s[4] = {INTC_MK(a, sph_trans_array_a), INTC_MK(b, sph_trans_array_b), INTC_MK(c, sph_trans_array_c),
+                     INTC_MK(d, sph_trans_array_d)};
+    return per_quartet_noalloc(s, output_array);
 }
 
 INTC_API int intc_eri_cont_cart_main(INTC_SHELL_PARAMS(a), INTC_SHELL_PARAMS(b), INTC_SHELL_PARAMS(c),

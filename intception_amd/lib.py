@@ -28,6 +28,7 @@ SIGNATURES = [
     ("intc_eri_cont_sph_work_array_size", c_int, [c_int] * 12),
     ("intc_eri_cont_cart_work_array_size", c_int, [c_int] * 12),
     ("intc_eri_cont_sph_main", c_int, (_SHELL + [_dp]) * 4 + [_dp, _dp]),
+    ("intc_eri_cont_sph_main_noalloc", c_int, (_SHELL + [_dp]) * 4 + [_dp, _dp]),
     ("intc_eri_cont_cart_main", c_int, _SHELL * 4 + [_dp, _dp]),
     ("intc_plan_create", c_int, [ctypes.POINTER(c_void_p), c_int, c_int, _ip, _ip, _ip, _dp, _dp, _dp, c_ll, _ip,
                                  _llp]),
