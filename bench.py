@@ -71,6 +71,39 @@ def cpu_baseline(batch, n_sample, seed=7, target_s=12.0, chunk=2_000_000):
             "quartets_per_s": float(n_q / dt)}
 
 
+def cpu_baseline_c99(batch, n_sample, seed=7, target_s=12.0, chunk=2_000_000):
+    """The generator's C99 backend (libintc_c99.so: the reference's file set
+    for the spherical ERI class, -O3 -march=x86-64-v3, OpenMP over quartets)
+    timed like cpu_baseline, on random samples of the same workload."""
+    from intception_amd import build as bld, c99_baseline
+    if not os.path.exists(bld.C99_LIB):
+        return None
+    rng = np.random.default_rng(seed)
+    threads = min(16, os.cpu_count() or 1)
+    warm = batch.subset(np.sort(rng.choice(batch.n_quartet, size=min(500, batch.n_quartet), replace=False)))
+    c99_baseline.eri_batch(warm, threads)
+    n_q, n_i, dt = 0, 0, 0.0
+    n = min(n_sample, batch.n_quartet)
+    while True:
+        sub = batch.subset(np.sort(rng.choice(batch.n_quartet, size=n, replace=False)))
+        t0 = time.perf_counter()
+        c99_baseline.eri_batch(sub, threads)
+        dt += time.perf_counter() - t0
+        n_q += sub.n_quartet
+        n_i += sub.out_size
+        del sub
+        if dt >= target_s or n_q >= 4 * batch.n_quartet:
+            break
+        rate = n_q / dt
+        n = int(min(chunk, batch.n_quartet, max(1000, rate * (target_s - dt) * 1.05)))
+    return {"value": float(n_i / dt), "unit": "integrals/s", "cores": threads, "kind": "port",
+            "sample": "%d quartets drawn at random from the same workload in samples of <= %d (%.1f s of CPU "
+                      "work); this build's C99 backend (generator_options(backend='c99'), the reference's file "
+                      "set for the spherical ERI class, l <= 2), gcc %s, OpenMP schedule(dynamic,64)"
+                      % (n_q, max(chunk, n_sample), dt, bld.C99_FLAGS),
+            "quartets_per_s": float(n_q / dt)}
+
+
 def kernel_label(b):
     """Name of the kernel a spherical FP64 bin launches with the built-in
     matrices: the generated class kernel's `_bs` form, or the workgroup
@@ -398,10 +431,12 @@ def run_molecule(args, three_center=False):
                    "peak": 2500.0, "unit": "TFLOP/s",
                    "frac": router_flops / (ms_route * 1e-3) / 1e12 / 2500.0,
                    "note": "BF16 MFMA layers of the routing network over every quartet of the step"}
-    cpu = None
+    cpu = cpu99 = None
     if not args.no_cpu:
         cpu = cpu_baseline(batch, args.cpu_sample)
         log("cpu baseline:", cpu)
+        cpu99 = cpu_baseline_c99(batch, args.cpu_sample)
+        log("cpu baseline (C99 backend):", cpu99)
     n_int = batch.out_size
     res = {
         "metric": "contracted ERIs/s on 1 MI355X: FP64 vs mixed+learned at max err <=1e-10",
@@ -461,6 +496,7 @@ def run_molecule(args, three_center=False):
                                                               dom["ms"]),
                      "job_frac_plain_fp64": float(flops_q.sum() / (ms64 * 1e-3) / 1e12 / FP64_PEAK_TFLOPS)},
         "cpu_baseline": cpu,
+        "cpu_baseline_c99_backend": cpu99,
         "setup_s": {"router_train_and_calibrate": t_router, "molecule_and_screening": t_setup, "plan": t_plan},
         "bins": [{"cls": r["cls"], "nq": r["nq"], "ms": round(r["ms"], 4)} for r in per_bin],
     }
@@ -586,10 +622,12 @@ def run_config1(args):
                  "within_1e-12_rel_1e-14_abs": bool(np.all(err <= 1e-14 + 1e-12 * np.abs(ref)))}
         log("parity check:", check)
 
-    cpu = None
+    cpu = cpu99 = None
     if not args.no_cpu:
         cpu = cpu_baseline(batch, args.cpu_sample)
         log("cpu baseline:", cpu)
+        cpu99 = cpu_baseline_c99(batch, args.cpu_sample)
+        log("cpu baseline (C99 backend):", cpu99)
 
     ints_per_s = batch.out_size / (ms_step * 1e-3)
     res = {
@@ -622,6 +660,7 @@ def run_config1(args):
                      "job_achieved": total_flops / (ms_step * 1e-3) / 1e12,
                      "job_frac": total_flops / (ms_step * 1e-3) / 1e12 / FP64_PEAK_TFLOPS},
         "cpu_baseline": cpu,
+        "cpu_baseline_c99_backend": cpu99,
         "parity": check,
         "plan_s": t_plan,
         "ms_one_launch_per_class_heuristic": ms_default,
@@ -709,6 +748,7 @@ def run_config2(args):
     dom = max(per_bin, key=lambda r: r["ms"])
     achieved = dom["flops"] / (dom["ms"] * 1e-3) / 1e12
     cpu = None if args.no_cpu else cpu_baseline(batch, 2000)
+    cpu99 = None
     if cpu:
         log("cpu baseline:", cpu)
     n_int = batch.out_size
@@ -730,6 +770,7 @@ def run_config2(args):
                      "kernel": kernel_label(dom), "kernel_ms": dom["ms"], "kernel_flops": dom["flops"],
                      "note": "dominant FP64 bin; flops = reference-scheme count (intception_amd/flops_model.py)"},
         "cpu_baseline": cpu,
+        "cpu_baseline_c99_backend": cpu99,
         "bins": [{"cls": r["cls"], "nq": r["nq"], "ms": round(r["ms"], 4)} for r in per_bin],
     }
     plan.close()

@@ -66,6 +66,48 @@ def _compile(src, deps, force=False):
     return obj, True
 
 
+C99_DIR = os.path.join(HERE, "_c99")
+C99_LIB = os.path.join(HERE, "libintc_c99.so")
+C99_FLAGS = "-O3 -march=x86-64-v3"
+
+
+def c99_full_name(lmax=2):
+    """<full> name the generator gives the spherical ERI class."""
+    from . import eri_script
+    from .generator import generator, generator_options
+    gen = generator(eri_script.make_eri(), opt=generator_options(spherical_transformed=True, global_user_lmax=lmax,
+                                                                 backend="c99"))
+    (name,) = gen._bindings
+    return name
+
+
+def build_c99(lmax=2, jobs=None, verbose=True):
+    """The generator's C99 backend for the spherical ERI class (shells up to
+    lmax), written as the reference's file set into _c99/ and built with its
+    own Makefile (libintception.a), then linked with the OpenMP batch driver
+    (csrc/intc_c99_batch.c) into libintc_c99.so: the CPU baseline that runs
+    this build's generated C beside the GPU path."""
+    from . import c99gen
+    name = c99_full_name(lmax)
+    c99gen.write(C99_DIR, name, True, lmax)
+    jobs = jobs or min(16, os.cpu_count() or 4)
+    r = subprocess.run(["make", "-s", "-j%d" % jobs, "-C", C99_DIR, "COPT=" + C99_FLAGS], capture_output=True,
+                       text=True)
+    if r.returncode != 0:
+        raise RuntimeError("C99 backend build failed:\n" + r.stderr[-4000:])
+    cmd = ["gcc", "-std=c99", "-fPIC", "-shared", "-fopenmp"] + C99_FLAGS.split() + [
+        "-I", C99_DIR, '-DINTC_C99_HEADER="%s.h"' % name, "-DINTC_C99_MAIN=%s_main" % name,
+        "-o", C99_LIB + ".tmp", os.path.join(CSRC, "intc_c99_batch.c"),
+        "-Wl,--whole-archive", os.path.join(C99_DIR, "libintception.a"), "-Wl,--no-whole-archive", "-lm"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("C99 batch driver link failed:\n" + r.stderr[-4000:])
+    os.replace(C99_LIB + ".tmp", C99_LIB)
+    if verbose:
+        print("libintc_c99.so: %s (C99 backend, l <= %d, %s)" % (name, lmax, C99_FLAGS))
+    return C99_LIB
+
+
 def build(lmax=2, jobs=None, verbose=True, force=False):
     """Generate the class kernels, compile every object (all of them when
     ``force``, else only those whose sources or flags changed) and link

@@ -52,11 +52,16 @@ def _prune(lines, roots):
     return keep[::-1]
 
 
-def _class_fn(cls):
-    return "eri_c_%d%d%d%d" % cls
+def _class_fn(cls, prefix="eri"):
+    return "%s_c_%d%d%d%d" % ((prefix,) + tuple(cls))
 
 
-def emit_class(cls):
+CLASS_ARGS = ("const double *xa, const double *ka, int na, const double *A, const double *xb, const double *kb, "
+              "int nb, const double *B, const double *xc, const double *kc, int nc, const double *C, "
+              "const double *xd, const double *kd, int nd, const double *D, double *out")
+
+
+def emit_class(cls, prefix="eri"):
     """Serial C for one canonical class: contracted Cartesian block
     out[a][b][c][d] of shells (A, B, C, D) in canonical order."""
     LA, LB, LC, LD = cls
@@ -64,7 +69,7 @@ def emit_class(cls):
     M = LX + LR
     L = []
     o = L.append
-    o("static void %s(const double *xa, const double *ka, int na, const double *A," % _class_fn(cls))
+    o("void %s(const double *xa, const double *ka, int na, const double *A," % _class_fn(cls, prefix))
     o("    const double *xb, const double *kb, int nb, const double *B,")
     o("    const double *xc, const double *kc, int nc, const double *C,")
     o("    const double *xd, const double *kd, int nd, const double *D, double *out)")
@@ -90,7 +95,7 @@ def emit_class(cls):
     o("            const double os = 1.0 / (zeta + eta), rho = zeta * eta * os;")
     o("            const double PQx = Px - Qx, PQy = Py - Qy, PQz = Pz - Qz;")
     o("            double F[%d];" % (M + 1))
-    o("            intc_c99_boys(%d, rho * (PQx * PQx + PQy * PQy + PQz * PQz), F);" % M)
+    o("            intception_boys(F, 1, 0, %d, rho * (PQx * PQx + PQy * PQy + PQz * PQz));" % M)
     o("            const double pref = %r * oz * oe * sqrt(os) * kab * kcd;" % TWO_PI52)
     o("            const double PAx = Px - A[0], PAy = Py - A[1], PAz = Pz - A[2];")
     o("            const double WPx = -eta * os * PQx, WPy = -eta * os * PQy, WPz = -eta * os * PQz;")
@@ -213,44 +218,243 @@ def _cheb_rows(mmax):
     return [bt.cheb_interval_poly(m, k) for m in range(mmax + 1) for k in range(bt.NINT)]
 
 
-def _boys_source(mmax):
+def _boys_files(mmax):
+    """boys_function.{h,c}: intception_boys with the reference's interface
+    (src/intception/generator/callables/generator_boys_function.py:98-205:
+    f[m * ifskip] = F_m(x) for nmin <= m <= nmax); the values come from this
+    build's scheme, a degree-DEG polynomial per unit interval of [0, NINT) for
+    F_nmax and downward recursion, the asymptotic F_0 and upward recursion
+    beyond (coefficients from F_n(x) = gamma(n + 1/2, x) / (2 x^(n + 1/2)),
+    40-digit arithmetic)."""
     from . import boys_tables as bt
     rows = _cheb_rows(mmax)
-    L = ["/* F_m(x) for m <= %d: degree-%d polynomial per unit interval of [0, %d) for F_M, downward"
-         % (mmax, bt.DEG, bt.NINT),
-         " * recursion; beyond, the asymptotic F_0 and upward recursion (coefficients from the",
-         " * definition F_n(x) = gamma(n + 1/2, x) / (2 x^(n + 1/2)), 40-digit arithmetic). */",
-         "static const double intc_c99_cheb[%d] = {" % (len(rows) * (bt.DEG + 1))]
+    h = "\n".join([
+        "/* Generated by intception_amd (c99 backend): Boys function F_m(x), m <= %d. */" % mmax,
+        "#ifndef INTCEPTION_BOYS_FUNCTION_H", "#define INTCEPTION_BOYS_FUNCTION_H",
+        "/* f[m * ifskip] = F_m(x) for nmin <= m <= nmax (nmax <= %d) */" % mmax,
+        "void intception_boys(double *f, const int ifskip, const int nmin, const int nmax, const double x);",
+        "#endif", ""])
+    L = ["/* Generated by intception_amd (c99 backend) -- do not edit.",
+         " * F_m(x) for m <= %d: degree-%d polynomial per unit interval of [0, %d) for F_nmax," % (mmax, bt.DEG,
+                                                                                               bt.NINT),
+         " * downward recursion; beyond, the asymptotic F_0 and upward recursion.  Coefficients from",
+         " * F_n(x) = gamma(n + 1/2, x) / (2 x^(n + 1/2)) in 40-digit arithmetic. */",
+         "#include <math.h>", '#include "boys_function.h"', "",
+         "static const double boys_cheb[%d] = {" % (len(rows) * (bt.DEG + 1))]
     for r in rows:
         L.append("    " + ", ".join(repr(float(v)) for v in r) + ",")
-    L += ["};",
-          "static void intc_c99_boys(int M, double T, double *F)",
+    L += ["};", "",
+          "void intception_boys(double *f, const int ifskip, const int nmin, const int nmax, const double x)",
           "{",
+          "    double F[%d];" % (mmax + 1),
+          "    double T = x > 0.0 ? x : 0.0;",
           "    int m;",
-          "    if (!(T > 0.0)) T = 0.0;",
           "    if (T < %d.0) {" % bt.NINT,
           "        const int k = (int)T;",
           "        const double t = 2.0 * (T - (double)k) - 1.0, ex = exp(-T);",
-          "        const double *c = intc_c99_cheb + (M * %d + k) * %d;" % (bt.NINT, bt.DEG + 1),
+          "        const double *c = boys_cheb + (nmax * %d + k) * %d;" % (bt.NINT, bt.DEG + 1),
           "        double v = c[%d];" % bt.DEG,
           "        int j;",
           "        for (j = %d; j >= 0; --j) v = v * t + c[j];" % (bt.DEG - 1),
-          "        F[M] = v;",
-          "        for (m = M - 1; m >= 0; --m) F[m] = (2.0 * T * F[m + 1] + ex) / (2.0 * m + 1.0);",
+          "        F[nmax] = v;",
+          "        for (m = nmax - 1; m >= 0; --m) F[m] = (2.0 * T * F[m + 1] + ex) / (2.0 * m + 1.0);",
           "    } else {",
           "        const double ex = exp(-T);",
           "        F[0] = 0.5 * sqrt(3.141592653589793238 / T);",
-          "        for (m = 0; m < M; ++m) F[m + 1] = ((2.0 * m + 1.0) * F[m] - ex) / (2.0 * T);",
+          "        for (m = 0; m < nmax; ++m) F[m + 1] = ((2.0 * m + 1.0) * F[m] - ex) / (2.0 * T);",
           "    }",
+          "    for (m = nmin; m <= nmax; ++m) f[m * ifskip] = F[m];",
           "}", ""]
-    return "\n".join(L)
+    return h, "\n".join(L)
 
 
-def emit_c99(full_name, spherical, lmax, classes=None):
-    """(c_source, h_source) for the ERI class with shells up to lmax (every
-    canonical class, or only `classes`; the others are refused at run time)."""
+INDEX_CHANGES = (-2, -1, 1)   # the ERI script's index changes: VRR a-1, a-2 (c-1, c-2); HRR a+1, b-1
+
+
+def _global_data_files(lmax):
+    """global_data.{h,c}: the reference's shared tables
+    (src/intception/generator/generator.py:678-753, 1019-1157), computed here:
+    ccs[n][3] the Cartesian exponents of flat component n (x^l first ... z^l
+    last, every l <= 2 lmax, the VRR's highest level); jump[n][c][3] the change
+    of the flat index when component d of n changes by INDEX_CHANGES[c] (0 where
+    that would be negative); hrr_layer[la][lb] the largest HGP layer
+    max_lq (ncc0(la + lb - lq) - ncc0(lb - 1)) ncc(lq), lq <= la."""
+    lint = 2 * lmax
+    comps = comps_upto(lint)
+    n = len(comps)
+    nch = len(INDEX_CHANGES)
+    h = "\n".join([
+        "/* Generated by intception_amd (c99 backend): shared index tables. */",
+        "#ifndef INTCEPTION_GLOBAL_DATA_H", "#define INTCEPTION_GLOBAL_DATA_H",
+        "#define INTCEPTION_NCC %d" % n, "#define INTCEPTION_NCHANGES %d" % nch,
+        "#define INTCEPTION_HRR_LA %d" % (lint - lmax + 1), "#define INTCEPTION_HRR_LB %d" % (lmax + 1),
+        "/* Cartesian exponents of flat component n (x^l first, ..., z^l last), l <= %d */" % lint,
+        "extern const int ccs[INTCEPTION_NCC][3];",
+        "/* flat-index change of component n when its d exponent changes by %s (0 if negative) */"
+        % ", ".join(str(c) for c in INDEX_CHANGES),
+        "extern const int jump[INTCEPTION_NCC][INTCEPTION_NCHANGES][3];",
+        "/* largest HGP HRR layer for final angular momenta (la, lb) */",
+        "extern const int hrr_layer[INTCEPTION_HRR_LA][INTCEPTION_HRR_LB];", "#endif", ""])
+    L = ["/* Generated by intception_amd (c99 backend) -- do not edit. */", '#include "global_data.h"', "",
+         "const int ccs[INTCEPTION_NCC][3] = {"]
+    L += ["    {%d, %d, %d}," % c for c in comps]
+    L += ["};", "", "const int jump[INTCEPTION_NCC][INTCEPTION_NCHANGES][3] = {"]
+    for c in comps:
+        rows = []
+        for ch in INDEX_CHANGES:
+            vals = []
+            for d in range(3):
+                t = list(c)
+                t[d] += ch
+                vals.append(cidx(tuple(t)) - cidx(c) if t[d] >= 0 else 0)
+            rows.append("{%d, %d, %d}" % tuple(vals))
+        L.append("    {%s}," % ", ".join(rows))
+    L += ["};", "", "const int hrr_layer[INTCEPTION_HRR_LA][INTCEPTION_HRR_LB] = {"]
+    for la in range(lint - lmax + 1):
+        vals = [max((ncc0(la + lb - lq) - ncc0(lb - 1)) * ncc(lq) for lq in range(la + 1)) for lb in range(lmax + 1)]
+        L.append("    {%s}," % ", ".join(str(v) for v in vals))
+    L += ["};", ""]
+    return h, "\n".join(L)
+
+
+GLOBAL_FUNCTIONS_H = """/* Generated by intception_amd (c99 backend): shared transforms. */
+#ifndef INTCEPTION_GLOBAL_FUNCTIONS_H
+#define INTCEPTION_GLOBAL_FUNCTIONS_H
+/* HGP HRR step on contracted values, (a, b) = (a + 1_d, b - 1_d) + Rfromto_d (a, b - 1_d),
+ * d the first non-zero component of b: for every b at level loop_l_move_to (x^l first ...),
+ * every a at levels l_move_from..lmax_move_from and every pair of contractions,
+ *   work[iwrk0 + ib iskip_move_to0 + ia iskip_move_from0 + ka iskip_cont_move_from0 + kb iskip_cont_move_to0]
+ *   = work[iwrk1 + (b - 1_d) iskip_move_to1 + (a + 1_d) iskip_move_from1 + ...]
+ *   + Rfromto[d] work[iwrk1 + (b - 1_d) iskip_move_to1 + a iskip_move_from1 + ...],
+ * with a counted over the levels from l_move_from up and b - 1_d within level loop_l_move_to - 1. */
+void hrr(const double Rfromto[3], const int lmax_move_from, const int l_move_from, const int loop_l_move_to,
+         const int iskip_move_from0, const int iskip_move_to0, const int iskip_move_from1, const int iskip_move_to1,
+         const int ncont_move_from, const int ncont_move_to, const int iskip_cont_move_from0,
+         const int iskip_cont_move_to0, const int iskip_cont_move_from1, const int iskip_cont_move_to1,
+         double *work_array, const int iwrk0, const int iwrk1);
+/* work[iwrk0 + ipre + icont n_pre] = sum_iprim work[iwrk1 + ipre + iprim iskip_prim]
+ *                                    * contract_array[iprim + icont n_prim] */
+void contract(double *work_array, const int iwrk0, const int iwrk1, const double *contract_array, const int n_pre,
+              const int n_prim, const int n_cont, const int iskip_prim);
+/* work[iwrk0 + ipre + isph n_pre] = sum_icart work[iwrk1 + ipre + icart iskip_cart]
+ *                                   * sph_trans_array[icart + isph n_cart] */
+void sph_trans(double *work_array, const int iwrk0, const int iwrk1, const double *sph_trans_array,
+               const int n_pre, const int n_cart, const int n_sph, const int iskip_cart);
+#endif
+"""
+
+GLOBAL_FUNCTIONS_C = """/* Generated by intception_amd (c99 backend) -- do not edit.
+ * The reference's shared functions (hrr, contract, sph_trans), same arguments and meaning
+ * (src/intception/generator/callables/generator_hrr_function.py:227-435,
+ * generator_contract_function.py:69-125, generator_sph_trans_function.py:69-125). */
+#include "global_functions.h"
+
+static int gf_ncc0(int l) { return l < 0 ? 0 : (l + 1) * (l + 2) * (l + 3) / 6; }
+/* position of (x, y, z) among the components of its level (x^l first) */
+static int gf_within(int y, int z) { return (y + z) * (y + z + 1) / 2 + z; }
+
+void hrr(const double Rfromto[3], const int lmax_move_from, const int l_move_from, const int loop_l_move_to,
+         const int iskip_move_from0, const int iskip_move_to0, const int iskip_move_from1, const int iskip_move_to1,
+         const int ncont_move_from, const int ncont_move_to, const int iskip_cont_move_from0,
+         const int iskip_cont_move_to0, const int iskip_cont_move_from1, const int iskip_cont_move_to1,
+         double *work_array, const int iwrk0, const int iwrk1)
+{
+    const int lt = loop_l_move_to, base = gf_ncc0(l_move_from - 1);
+    int kt, kf, bx, by, it;
+    for (kt = 0; kt < ncont_move_to; ++kt) {
+        it = 0;
+        for (bx = lt; bx >= 0; --bx)
+            for (by = lt - bx; by >= 0; --by, ++it) {
+                const int bz = lt - bx - by;
+                /* build direction: the first non-zero component of b */
+                const int d = bx > 0 ? 0 : (by > 0 ? 1 : 2);
+                const int im1 = gf_within(by - (d == 1), bz - (d == 2));
+                const double R = Rfromto[d];
+                for (kf = 0; kf < ncont_move_from; ++kf) {
+                    const int o0 = iwrk0 + it * iskip_move_to0 + kf * iskip_cont_move_from0 + kt * iskip_cont_move_to0;
+                    const int o1 = iwrk1 + im1 * iskip_move_to1 + kf * iskip_cont_move_from1 +
+                                   kt * iskip_cont_move_to1;
+                    int l, ax, ay, ia = 0;
+                    for (l = l_move_from; l <= lmax_move_from; ++l)
+                        for (ax = l; ax >= 0; --ax)
+                            for (ay = l - ax; ay >= 0; --ay, ++ia) {
+                                const int az = l - ax - ay;
+                                /* a + 1_d sits at level l + 1 */
+                                const int ip = gf_ncc0(l) + gf_within(ay + (d == 1), az + (d == 2)) - base;
+                                work_array[o0 + ia * iskip_move_from0] =
+                                    work_array[o1 + ip * iskip_move_from1] + R * work_array[o1 + ia * iskip_move_from1];
+                            }
+                }
+            }
+    }
+}
+
+void contract(double *work_array, const int iwrk0, const int iwrk1, const double *contract_array, const int n_pre,
+              const int n_prim, const int n_cont, const int iskip_prim)
+{
+    int k, p, i;
+    for (k = 0; k < n_cont; ++k) {
+        double *dst = work_array + iwrk0 + k * n_pre;
+        for (i = 0; i < n_pre; ++i) dst[i] = 0.0;
+        for (p = 0; p < n_prim; ++p) {
+            const double w = contract_array[p + k * n_prim];
+            const double *src = work_array + iwrk1 + p * iskip_prim;
+            for (i = 0; i < n_pre; ++i) dst[i] += src[i] * w;
+        }
+    }
+}
+
+void sph_trans(double *work_array, const int iwrk0, const int iwrk1, const double *sph_trans_array,
+               const int n_pre, const int n_cart, const int n_sph, const int iskip_cart)
+{
+    int s, c, i;
+    for (s = 0; s < n_sph; ++s) {
+        double *dst = work_array + iwrk0 + s * n_pre;
+        for (i = 0; i < n_pre; ++i) dst[i] = 0.0;
+        for (c = 0; c < n_cart; ++c) {
+            const double w = sph_trans_array[c + s * n_cart];
+            const double *src = work_array + iwrk1 + c * iskip_cart;
+            if (w == 0.0) continue;
+            for (i = 0; i < n_pre; ++i) dst[i] += src[i] * w;
+        }
+    }
+}
+"""
+
+MAKEFILE = """# Generated by intception_amd (c99 backend): builds lib{lib}.a from every .c file here,
+# as examples/Makefile of the reference does for its output directory.
+CC ?= gcc
+CSTD := -std=c99
+COPT ?= -O3
+WARN := -Wall
+SRC := $(wildcard *.c)
+OBJ := $(SRC:.c=.o)
+lib{lib}.a : $(OBJ)
+\tar rcs $@ $(OBJ)
+%.o : %.c
+\t$(CC) $(CSTD) $(COPT) $(WARN) -fPIC -MMD -c $< -o $@
+.PHONY : clean
+clean :
+\trm -f $(OBJ) $(OBJ:.o=.d) lib{lib}.a
+-include $(OBJ:.o=.d)
+"""
+
+
+def emit_file_set(full_name, spherical, lmax, classes=None, lib="intception"):
+    """The reference's output file set for the ERI class
+    (src/intception/generator/generator.py:1504-1702), from this build's
+    recurrence structure: global_data.{c,h}, boys_function.{c,h},
+    global_functions.{c,h}, <full>.{c,h} (the <full>_main dispatcher and
+    <full>_work_array_size), one <full>_<abcd>.c per canonical class (so a
+    parallel make compiles them side by side, declared in <full>_classes.h),
+    and a Makefile for lib<lib>.a.  Returns {file name: text}."""
     classes = canonical_classes(lmax) if classes is None else sorted(canonical(*c)[0] for c in classes)
     kind = "sph" if spherical else "cart"
+    files = {}
+    files["global_data.h"], files["global_data.c"] = _global_data_files(lmax)
+    files["boys_function.h"], files["boys_function.c"] = _boys_files(4 * lmax)
+    files["global_functions.h"], files["global_functions.c"] = GLOBAL_FUNCTIONS_H, GLOBAL_FUNCTIONS_C
+    files["Makefile"] = MAKEFILE.replace("{lib}", lib)
     args = []
     for X in "abcd":
         args.append("const double *x{0}, const double c{0}[3], const int l{0}, const int n{0}, const int iskip{0}, "
@@ -263,31 +467,39 @@ def emit_c99(full_name, spherical, lmax, classes=None):
                   "const int nb_prim, const int nc_prim, const int nd_prim, const int na_cont, const int nb_cont, "
                   "const int nc_cont, const int nd_cont)" % full_name)
     guard = full_name.upper() + "_H"
-    h = "\n".join([
+    files[full_name + ".h"] = "\n".join([
         "/* Generated by intception_amd (c99 backend) for the contracted ERI class, %s output, l <= %d. */"
         % ("spherical" if spherical else "Cartesian", lmax),
         "#ifndef %s" % guard, "#define %s" % guard, proto_main + ";", proto_size + ";", "#endif", ""])
+    cguard = full_name.upper() + "_CLASSES_H"
+    files[full_name + "_classes.h"] = "\n".join(
+        ["/* Generated by intception_amd (c99 backend): canonical-class functions of %s. */" % full_name,
+         "#ifndef %s" % cguard, "#define %s" % cguard,
+         "/* contracted Cartesian block out[a][b][c][d] of shells A, B, C, D in canonical order",
+         " * (la >= lb, lc >= ld, (la + lb, la) <= (lc + ld, lc)); coefficients ka.. normalised */"]
+        + ["void %s(%s);" % (_class_fn(c, full_name), CLASS_ARGS) for c in classes] + ["#endif", ""])
+    for cls in classes:
+        files["%s_%d%d%d%d.c" % ((full_name,) + tuple(cls))] = "\n".join([
+            "/* Generated by intception_amd (c99 backend) -- do not edit.",
+            " * Contracted ERI class (%d %d | %d %d): Obara-Saika VRR on a then c, HGP HRR on the" % tuple(cls),
+            " * contracted ket (c, d) and bra (a, b). */",
+            "#include <math.h>", '#include "boys_function.h"', '#include "%s_classes.h"' % full_name, "",
+            emit_class(cls, full_name)])
     S = ["/* Generated by intception_amd (c99 backend) -- do not edit.",
-         " * Contracted ERI (ab|cd), Obara-Saika VRR on a then c, HGP HRR on the contracted",
-         " * ket (c, d) and bra (a, b), %s output, shells up to l = %d. */" % (kind, lmax),
-         "#include <math.h>", "#include <stdio.h>", '#include "%s.h"' % full_name, "",
-         _boys_source(4 * lmax)]
+         " * <full>_main and <full>_work_array_size of the contracted ERI, %s output, shells up to l = %d. */"
+         % (kind, lmax),
+         "#include <stdio.h>", '#include "global_data.h"', '#include "global_functions.h"',
+         '#include "%s.h"' % full_name, '#include "%s_classes.h"' % full_name, "",
+         "typedef void (*eri_c_fn)(const double *, const double *, int, const double *, const double *, "
+         "const double *, int, const double *, const double *, const double *, int, const double *, "
+         "const double *, const double *, int, const double *, double *);",
+         "static eri_c_fn class_fn(int la, int lb, int lc, int ld)", "{"]
     for cls in classes:
-        S.append(emit_class(cls))
-    S.append("typedef void (*eri_c_fn)(const double *, const double *, int, const double *, const double *, "
-             "const double *, int, const double *, const double *, const double *, int, const double *, "
-             "const double *, const double *, int, const double *, double *);")
-    S.append("static eri_c_fn class_fn(int la, int lb, int lc, int ld)")
-    S.append("{")
-    for cls in classes:
-        S.append("    if (la == %d && lb == %d && lc == %d && ld == %d) return %s;" % (cls + (_class_fn(cls),)))
-    S.append("    return 0;")
-    S.append("}")
-    S.append("")
-    S.append("static int c99_ncc(int l) { return (l + 1) * (l + 2) / 2; }")
-    S.append("static int c99_ncc0(int l) { return l < 0 ? 0 : (l + 1) * (l + 2) * (l + 3) / 6; }")
-    S.append("")
-    # dispatcher
+        S.append("    if (la == %d && lb == %d && lc == %d && ld == %d) return %s;" % (cls + (_class_fn(cls, full_name),)))
+    S += ["    return 0;", "}", "",
+          "static int c99_ncc(int l) { return (l + 1) * (l + 2) / 2; }",
+          "static int c99_ncc0(int l) { return l < 0 ? 0 : (l + 1) * (l + 2) * (l + 3) / 6; }", ""]
+    nbuf = ncc(lmax) ** 4
     main = [proto_main, "{",
             "    const double *xs[4] = {xa, xb, xc, xd}, *cs[4] = {ca, cb, cc, cd};",
             "    const double *ks[4] = {contract_array_a, contract_array_b, contract_array_c, contract_array_d};",
@@ -303,7 +515,8 @@ def emit_c99(full_name, spherical, lmax, classes=None):
         "    int p[4], q, k0, k1, k2, k3;",
         "    int bra0 = 0, bra1 = 1, ket0 = 2, ket1 = 3;",
         "    eri_c_fn f;",
-        "    double blk[%d], tmp[%d];" % (ncc(lmax) ** 4, ncc(lmax) ** 4),
+        "    /* blk: the canonical block; tmp: caller order; two buffers for the transforms */",
+        "    double blk[%d], tmp[%d];" % (nbuf, 2 * nbuf),
         "    (void)work_array;",
         "    /* canonical class: la >= lb, lc >= ld, (la+lb, la) <= (lc+ld, lc) in the bra */",
         "    if (ls[0] < ls[1]) { bra0 = 1; bra1 = 0; }",
@@ -329,6 +542,7 @@ def emit_c99(full_name, spherical, lmax, classes=None):
         "    for (k2 = 0; k2 < nco[2]; ++k2) for (k3 = 0; k3 < nco[3]; ++k3) {",
         "        const int kk[4] = {k0, k1, k2, k3};",
         "        int nct[4], st[4], i0, i1, i2, i3, x;",
+        "        double *res = tmp;",
         "        f(xs[p[0]], ks[p[0]] + kk[p[0]] * np[p[0]], np[p[0]], cs[p[0]],",
         "          xs[p[1]], ks[p[1]] + kk[p[1]] * np[p[1]], np[p[1]], cs[p[1]],",
         "          xs[p[2]], ks[p[2]] + kk[p[2]] * np[p[2]], np[p[2]], cs[p[2]],",
@@ -344,29 +558,23 @@ def emit_c99(full_name, spherical, lmax, classes=None):
         "        }"]
     if spherical:
         main += [
-            "        /* out = in . S per index, last index first (sph_trans, S[icart + isph * ncart]) */",
+            "        /* caller's S on each index with sph_trans, one call per slab of the slower",
+            "         * indices: a block [outer][n_cart][inner] becomes [outer][n_sph][inner] */",
             "        {",
-            "            int dims[4], ax, a0, ai;",
-            "            double *src = tmp, *dst = blk, *sw;",
+            "            int dims[4], ax, o, y, src = 0, dst = %d;" % nbuf,
             "            for (x = 0; x < 4; ++x) dims[x] = nct[x];",
             "            for (ax = 3; ax >= 0; --ax) {",
             "                const int nsp = 2 * ls[ax] + 1, nca = dims[ax];",
-            "                int inner = 1, outer = 1, y;",
+            "                int inner = 1, outer = 1;",
             "                for (y = ax + 1; y < 4; ++y) inner *= dims[y];",
             "                for (y = 0; y < ax; ++y) outer *= dims[y];",
-            "                for (a0 = 0; a0 < outer; ++a0)",
-            "                    for (ai = 0; ai < nsp; ++ai)",
-            "                        for (y = 0; y < inner; ++y) {",
-            "                            double s = 0.0;",
-            "                            int ic;",
-            "                            for (ic = 0; ic < nca; ++ic)",
-            "                                s += src[(a0 * nca + ic) * inner + y] * ss[ax][ic + ai * nca];",
-            "                            dst[(a0 * nsp + ai) * inner + y] = s;",
-            "                        }",
+            "                for (o = 0; o < outer; ++o)",
+            "                    sph_trans(tmp, dst + o * nsp * inner, src + o * nca * inner, ss[ax], inner, nca, nsp,",
+            "                              inner);",
             "                dims[ax] = nsp;",
-            "                sw = src; src = dst; dst = sw;",
+            "                y = src; src = dst; dst = y;",
             "            }",
-            "            /* four passes: the result is back in tmp */",
+            "            res = tmp + src;",
             "        }"]
     main += [
         "        {",
@@ -375,19 +583,15 @@ def emit_c99(full_name, spherical, lmax, classes=None):
         "            for (i0 = 0; i0 < n0; ++i0) for (i1 = 0; i1 < n1; ++i1)",
         "            for (i2 = 0; i2 < n2; ++i2) for (i3 = 0; i3 < n3; ++i3, ++t)",
         "                output_array[i0 * isk[0] + kk[0] * iskc[0] + i1 * isk[1] + kk[1] * iskc[1] +",
-        "                             i2 * isk[2] + kk[2] * iskc[2] + i3 * isk[3] + kk[3] * iskc[3]] = tmp[t];",
+        "                             i2 * isk[2] + kk[2] * iskc[2] + i3 * isk[3] + kk[3] * iskc[3]] = res[t];",
         "        }",
         "    }",
         "}"]
     S += main
-    S += ["", "static long c99_hrr_layer(int la, int lb)", "{",
-          "    long best = 0, s;", "    int lq;",
-          "    for (lq = 0; lq <= la; ++lq) {",
-          "        s = (long)(c99_ncc0(la + lb - lq) - c99_ncc0(lb - 1)) * c99_ncc(lq);",
-          "        if (s > best) best = s;", "    }", "    return best;", "}", "",
-          proto_size, "{",
+    S += ["", proto_size, "{",
           "    /* the reference's formula: 2-layer VRR buffer, early contractions in either order,",
-          "     * HRR layer, late transform (contracted_vrr_hrr_with_auxiliary.py:248-351) */",
+          "     * HRR layer (hrr_layer of global_data), late transform",
+          "     * (contracted_vrr_hrr_with_auxiliary.py:248-351) */",
           "    const int L = la + lb, M = L + lc + ld;",
           "    const long nvrr = (long)c99_ncc0(L) * c99_ncc0(lc) * c99_ncc0(ld);",
           "    const long nprim = (long)na_prim * nb_prim * nc_prim * nd_prim;",
@@ -395,8 +599,10 @@ def emit_c99(full_name, spherical, lmax, classes=None):
           "    const long ncont = (long)na_cont * nb_cont * nc_cont * nd_cont;",
           "    const int nc_c = %s, nc_d = %s;" % (("2 * lc + 1", "2 * ld + 1") if spherical else
                                                 ("c99_ncc(lc)", "c99_ncc(ld)")),
-          "    long mem = 2 * (nvrr * nprim + M + 1), nmax = nprim, late;",
+          "    long mem = 2 * (nvrr * nprim + M + 1), nmax = nprim, late, layer;",
           "    int dir, s, cur[4];",
+          "    if (lb < 0 || lb >= INTCEPTION_HRR_LA || la < 0 || la >= INTCEPTION_HRR_LB) return -1;",
+          "    layer = hrr_layer[lb][la];",
           "    for (dir = 0; dir < 2; ++dir) {",
           "        for (s = 0; s < 4; ++s) cur[s] = np[s];",
           "        for (s = 0; s < 4; ++s) {",
@@ -408,20 +614,26 @@ def emit_c99(full_name, spherical, lmax, classes=None):
           "        }",
           "    }",
           "    if (2 * nvrr * nmax > mem) mem = 2 * nvrr * nmax;",
-          "    if (2 * c99_hrr_layer(lb, la) * nc_c * nc_d * ncont > mem) mem = 2 * c99_hrr_layer(lb, la) * nc_c * nc_d"
-          " * ncont;",
+          "    if (2 * layer * nc_c * nc_d * ncont > mem) mem = 2 * layer * nc_c * nc_d * ncont;",
           "    late = %s;" % ("2L * c99_ncc(la) * c99_ncc(lb) * c99_ncc(lc) * c99_ncc(ld) * ncont" if spherical else
                                "2L * c99_ncc(la) * c99_ncc(lb) * nc_c * nc_d * ncont"),
           "    if (late > mem) mem = late;",
           "    return (int)mem;", "}", ""]
-    return "\n".join(S), h
+    files[full_name + ".c"] = "\n".join(S)
+    return files
 
 
-def write(outdir, full_name, spherical, lmax, classes=None):
+def write(outdir, full_name, spherical, lmax, classes=None, lib="intception"):
+    """Write the file set of emit_file_set into outdir; returns the paths of
+    its .c files."""
     os.makedirs(outdir, exist_ok=True)
-    c, h = emit_c99(full_name, spherical, lmax, classes)
-    with open(os.path.join(outdir, full_name + ".c"), "w") as fh:
-        fh.write(c)
-    with open(os.path.join(outdir, full_name + ".h"), "w") as fh:
-        fh.write(h)
-    return os.path.join(outdir, full_name + ".c")
+    out = []
+    for name, text in emit_file_set(full_name, spherical, lmax, classes, lib).items():
+        path = os.path.join(outdir, name)
+        old = open(path).read() if os.path.exists(path) else None
+        if old != text:
+            with open(path, "w") as fh:
+                fh.write(text)
+        if name.endswith(".c"):
+            out.append(path)
+    return sorted(out)

@@ -532,7 +532,9 @@ class generator:
         d = self._opt.output_directory()
         os.makedirs(d, exist_ok=True)
         if self._opt.backend() == "c99":
-            # CPU source of each ERI class (<full>.c, <full>.h), reference interface
+            # CPU C99 of each ERI class in the reference's file set (global_data,
+            # boys_function, global_functions, <full> and its class files, a
+            # Makefile for libintception.a), reference interface
             from . import c99gen
             written = {}
             for n, b in self._bindings.items():

@@ -11,9 +11,11 @@ work-array size formula.
 """
 import ctypes
 import itertools
+import os
 import subprocess
 
 import numpy as np
+import pytest
 
 from conftest import assert_parity
 from intception_amd import basis, c99gen, eri_script
@@ -23,10 +25,14 @@ from oracle import oracle
 _dp = ctypes.POINTER(ctypes.c_double)
 
 
-def _compile(c_path, tmp_path, name):
+def _compile(c_paths, tmp_path, name):
+    """The emitted file set through its own Makefile (static library), then
+    linked into a shared object for ctypes."""
+    d = os.path.dirname(c_paths[0])
+    subprocess.run(["make", "-s", "-j8", "-C", d, "COPT=-O1", "WARN=-Wall -Werror"], check=True)
     so = str(tmp_path / (name + ".so"))
-    subprocess.run(["gcc", "-std=c99", "-O1", "-fPIC", "-shared", "-Wall", "-Werror", "-o", so, c_path, "-lm"],
-                   check=True)
+    subprocess.run(["gcc", "-shared", "-o", so, "-Wl,--whole-archive", os.path.join(d, "libintception.a"),
+                    "-Wl,--no-whole-archive", "-lm"], check=True)
     return ctypes.CDLL(so)
 
 
@@ -73,11 +79,14 @@ def test_generator_emits_c99_matching_oracle(tmp_path):
     rng = np.random.default_rng(21)
     for spherical in (False, True):
         gen = generator(eri_script.make_eri(), opt=generator_options(
-            output_directory=str(tmp_path), contracted=True, spherical_transformed=spherical,
+            output_directory=str(tmp_path / ("sph" if spherical else "cart")), contracted=True,
+            spherical_transformed=spherical,
             global_user_lmax=1, backend="c99"))
         written = gen.out()
-        (name, c_path), = written.items()
-        L = _compile(c_path, tmp_path, name)
+        (name, c_paths), = written.items()
+        assert {os.path.basename(p) for p in c_paths} >= {"global_data.c", "boys_function.c", "global_functions.c",
+                                                          name + ".c"}
+        L = _compile(c_paths, tmp_path, name)
         fn = getattr(L, name + "_main")
         for ls in itertools.product(range(2), repeat=4):
             shells = _shells(rng, ls, ncont=(2, 1, 1, 1))
@@ -97,8 +106,114 @@ def test_c99_d_shell_classes(tmp_path):
     dispatcher's permutation to the canonical class)."""
     oracle.set_boys_mode(oracle.BOYS_EXACT)
     rng = np.random.default_rng(22)
-    c_path = c99gen.write(str(tmp_path), "eri_sel", True, 2, classes=[(1, 1, 2, 1), (2, 0, 1, 0)])
-    fn = getattr(_compile(c_path, tmp_path, "eri_sel"), "eri_sel_main")
+    c_paths = c99gen.write(str(tmp_path / "sel"), "eri_sel", True, 2, classes=[(1, 1, 2, 1), (2, 0, 1, 0)])
+    fn = getattr(_compile(c_paths, tmp_path, "eri_sel"), "eri_sel_main")
     for ls in [(1, 1, 2, 1), (2, 1, 1, 1), (1, 1, 1, 2), (2, 0, 1, 0), (0, 2, 0, 1), (1, 0, 2, 0)]:
         got, ref = _call(fn, _shells(rng, ls), True)
         assert_parity(got, ref, what="c99 %s" % (ls,))
+
+
+def _global_lib(tmp_path, lmax=2):
+    files = c99gen.emit_file_set("eri_g", False, lmax, classes=[(0, 0, 0, 0)])
+    d = tmp_path / "glob"
+    d.mkdir()
+    for n in ("global_data.c", "global_data.h", "global_functions.c", "global_functions.h"):
+        (d / n).write_text(files[n])
+    so = str(d / "libglob.so")
+    subprocess.run(["gcc", "-std=c99", "-O1", "-fPIC", "-shared", "-Wall", "-Werror", "-o", so, str(d / "global_data.c"),
+                    str(d / "global_functions.c")], check=True)
+    return ctypes.CDLL(so)
+
+
+def test_c99_global_data_tables(tmp_path):
+    """ccs is the component order of the reference (x^l first, dsl.py:988-1041);
+    jump[n][c][d] is the flat-index change of component n when its d exponent
+    changes by -2, -1, +1 (0 where negative); hrr_layer is the reference's
+    maximum HRR layer, the term that sets <full>_work_array_size for l > 0."""
+    lmax = 2
+    L = _global_lib(tmp_path, lmax)
+    comps = [c for l in range(2 * lmax + 1) for c in basis.cart_components(l)]
+    ccs = np.ctypeslib.as_array((ctypes.c_int * (3 * len(comps))).in_dll(L, "ccs")).reshape(-1, 3)
+    assert [tuple(r) for r in ccs] == [tuple(c) for c in comps]
+    idx = {tuple(c): i for i, c in enumerate(comps)}
+    jump = np.ctypeslib.as_array((ctypes.c_int * (9 * len(comps))).in_dll(L, "jump")).reshape(-1, 3, 3)
+    for n, c in enumerate(comps):
+        for k, ch in enumerate((-2, -1, 1)):
+            for d in range(3):
+                t = list(c)
+                t[d] += ch
+                want = 0 if t[d] < 0 else (idx[tuple(t)] - n if tuple(t) in idx else jump[n, k, d])
+                assert jump[n, k, d] == want
+    hl = np.ctypeslib.as_array((ctypes.c_int * ((lmax + 1) ** 2)).in_dll(L, "hrr_layer")).reshape(lmax + 1, -1)
+    n0 = lambda l: 0 if l < 0 else (l + 1) * (l + 2) * (l + 3) // 6
+    for la in range(lmax + 1):
+        for lb in range(lmax + 1):
+            assert hl[la, lb] == max((n0(la + lb - q) - n0(lb - 1)) * (q + 1) * (q + 2) // 2 for q in range(la + 1))
+
+
+def test_c99_contract_and_sph_trans(tmp_path):
+    L = _global_lib(tmp_path)
+    rng = np.random.default_rng(3)
+    n_pre, n_prim, n_cont, isk = 7, 4, 3, 9
+    w = rng.standard_normal(1000)
+    C = np.asfortranarray(rng.standard_normal((n_prim, n_cont)))
+    got = w.copy()
+    L.contract(got.ctypes.data_as(_dp), 500, 10, C.ctypes.data_as(_dp), n_pre, n_prim, n_cont, isk)
+    src = np.array([[w[10 + i + p * isk] for p in range(n_prim)] for i in range(n_pre)])
+    want = (src @ C).T.reshape(-1)
+    assert np.allclose(got[500:500 + n_pre * n_cont], want, rtol=1e-14, atol=0)
+    S = np.asfortranarray(basis.SPH[2])
+    got = w.copy()
+    L.sph_trans(got.ctypes.data_as(_dp), 600, 20, S.ctypes.data_as(_dp), n_pre, 6, 5, 8)
+    src = np.array([[w[20 + i + c * 8] for c in range(6)] for i in range(n_pre)])
+    assert np.allclose(got[600:600 + n_pre * 5], (src @ S).T.reshape(-1), rtol=1e-14, atol=1e-16)
+
+
+def test_c99_hrr_matches_oracle(tmp_path):
+    """hrr with the reference's argument meaning: from (e s|cd) blocks of the
+    oracle (|e| = la..la+lb on A, an s shell on B with b's exponents and
+    coefficients), lb calls build (a b|cd), which must equal the oracle's
+    (a b|cd) -- the HGP identity holds for any fixed contraction of B."""
+    L = _global_lib(tmp_path)
+    L.hrr.argtypes = [_dp] + [ctypes.c_int] * 13 + [_dp, ctypes.c_int, ctypes.c_int]
+    rng = np.random.default_rng(9)
+    oracle.set_boys_mode(oracle.BOYS_EXACT)
+    ncc = lambda l: (l + 1) * (l + 2) // 2
+    for la, lb, lc, ld in [(1, 1, 1, 0), (2, 1, 0, 0), (1, 2, 1, 1), (2, 2, 1, 0)]:
+        A, B, C, D = (rng.uniform(0, 3, size=3) for _ in range(4))
+        ex = [10 ** rng.uniform(-0.5, 1, size=2) for _ in range(4)]
+        co = [rng.uniform(0.2, 1, size=2) for _ in range(4)]
+        sh = lambda l, k, cen: {"l": l, "exps": ex[k], "coefs": co[k], "cen": cen}
+        ncd = ncc(lc) * ncc(ld)
+        L_ = la + lb
+        # level 0 of b: rows e over levels la..L (flat), columns cd
+        rows = np.concatenate([oracle.eri_quartet([sh(n, 0, A), sh(0, 1, B), sh(lc, 2, C), sh(ld, 3, D)], False)
+                               .reshape(ncc(n), ncd) for n in range(la, L_ + 1)])
+        R = (ctypes.c_double * 3)(*(A - B))
+        cur = rows.reshape(1, -1, ncd)           # [b][a][cd]
+        for lt in range(1, lb + 1):
+            nfrom = sum(ncc(n) for n in range(la, L_ - lt + 1))
+            nfrom1 = cur.shape[1]
+            work = np.zeros(cur.size + ncc(lt) * nfrom * ncd)
+            work[:cur.size] = cur.reshape(-1)
+            o0 = cur.size
+            for j in range(ncd):
+                L.hrr(R, L_ - lt, la, lt, ncd, nfrom * ncd, ncd, nfrom1 * ncd, 1, 1, 0, 0, 0, 0,
+                      work.ctypes.data_as(_dp), o0 + j, j)
+            cur = work[o0:].reshape(ncc(lt), nfrom, ncd)
+        got = np.transpose(cur, (1, 0, 2)).reshape(-1)   # [a][b][cd]
+        ref = oracle.eri_quartet([sh(la, 0, A), sh(lb, 1, B), sh(lc, 2, C), sh(ld, 3, D)], False)
+        assert_parity(got, ref, what="hrr (%d%d|%d%d)" % (la, lb, lc, ld))
+
+
+def test_c99_batch_library_matches_oracle():
+    """libintc_c99.so (the C99 backend for l <= 2, built by build(), the
+    bench's second CPU baseline): a config-1 style batch against the oracle."""
+    from intception_amd import build as bld, c99_baseline
+    if not os.path.exists(bld.C99_LIB):
+        pytest.skip("libintc_c99.so not built (intception_amd.build.build_c99)")
+    b = basis.config1(n_quartet=300, seed=5)
+    got = c99_baseline.eri_batch(b, 4)
+    oracle.set_boys_mode(oracle.BOYS_EXACT)
+    ref = oracle.eri_batch(b, True, basis.SPH)
+    assert_parity(got, ref, what="C99 backend batch vs oracle")
