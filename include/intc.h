@@ -71,7 +71,7 @@ int intc_eri_cont_sph_main(INTC_SHELL_ARGS(a), const double *sph_trans_array_a, 
  * intc_init() looks up, so intc_init() must have run on the current device
  * (-1 otherwise).  Limits: n_prim <= 8 and n_cont <= 2 per index, indices
  * with the same l pass the same sph_trans_array, all contraction blocks
- * together <= 1 033 408 doubles (-1 beyond).  Calls that overlap on one GPU
+ * together <= 1 033 376 doubles (-1 beyond).  Calls that overlap on one GPU
  * share the array and race. */
 int intc_eri_cont_sph_main_noalloc(INTC_SHELL_ARGS(a), const double *sph_trans_array_a, INTC_SHELL_ARGS(b),
                                    const double *sph_trans_array_b, INTC_SHELL_ARGS(c),
@@ -108,6 +108,22 @@ int intc_plan_create_pairs(intc_plan **plan, int spherical, int n_shell, const i
                            const int *shell_off, const double *shell_cen, const double *exps, const double *coefs,
                            int n_pair, const int *pair_shells, long long n_quartet, const int *quartet_pairs,
                            const long long *out_off);
+/* General contraction and caller strides, the per-quartet fields of the
+ * reference's <full>_main (n_cont, iskip, iskip_cont per index;
+ * contracted_general.py:44-65, 1808-1875) for a batch: shell s has
+ * shell_ncont[s] contractions, column k of its coefficients (normalised,
+ * n_prim x n_cont column-major, as contract_array) at coefs[coef_off[s] + k *
+ * shell_nprim[s]], exponents at exps[shell_off[s]]; quartet q writes element
+ * (ia, ib, ic, id) of contraction (ka, kb, kc, kd) at out[out_off[q] +
+ * sum_X(iX * iskip[4q+X] + kX * iskip_cont[4q+X])] (iskip in [0, 65535]).
+ * execute receives exps, coefs and shell_cen in this layout.  Routed
+ * execution (which zeroes dense skipped blocks) is refused on such plans;
+ * intc_plan_quartet_order lists the expanded (quartet, contractions) items. */
+int intc_plan_create_general(intc_plan **plan, int spherical, int n_shell, const int *shell_l,
+                             const int *shell_nprim, const int *shell_ncont, const int *shell_off,
+                             const int *coef_off, const double *shell_cen, const double *exps, const double *coefs,
+                             long long n_quartet, const int *quartets, const long long *out_off, const int *iskip,
+                             const int *iskip_cont);
 int intc_plan_num_bins(const intc_plan *plan);
 /* prec: 0 = FP64 kernel, 1 = kernel with FP32 recurrences (FP64 Boys,
  * prefactors, contraction sums and contracted HRR/transform). */

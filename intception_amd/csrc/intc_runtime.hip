@@ -116,11 +116,15 @@ void canonical(const int l[4], int cls[4], int perm[4])
 __global__ void __launch_bounds__(256) intc_build_pairs(
     long long n_pp, const int *__restrict__ pp_pair, const int *__restrict__ pair_sa,
     const int *__restrict__ pair_sb, const int *__restrict__ pair_off, const int *__restrict__ pair_nk,
-    const int *__restrict__ shell_off, const double *__restrict__ shell_cen,
-    const double *__restrict__ exps, const double *__restrict__ coefs,
+    const int *__restrict__ exp_off, const int *__restrict__ coef_off, const int *__restrict__ cen_idx,
+    const double *__restrict__ shell_cen, const double *__restrict__ exps, const double *__restrict__ coefs,
     double *__restrict__ pz, double *__restrict__ po2z, double *__restrict__ ppx, double *__restrict__ ppy,
     double *__restrict__ ppz, double *__restrict__ pkap, double *__restrict__ pairA, double *__restrict__ pairAB)
 {
+    // plan shell s: exponents at exps[exp_off[s]], coefficients (one
+    // contraction) at coefs[coef_off[s]], centre shell_cen[3 cen_idx[s]]; a
+    // plan over general contractions has one plan shell per (shell,
+    // contraction), all of them reading the caller's arrays
     long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n_pp) return;
     const int pr = pp_pair[t];
@@ -128,10 +132,11 @@ __global__ void __launch_bounds__(256) intc_build_pairs(
     const int local = (int)(t - pair_off[pr]);
     const int kb = pair_nk[pr];
     const int ia = local / kb, ib = local - ia * kb;
-    const double al = exps[shell_off[sa] + ia], be = exps[shell_off[sb] + ib];
-    const double ca = coefs[shell_off[sa] + ia], cb = coefs[shell_off[sb] + ib];
-    const double Ax = shell_cen[3 * sa], Ay = shell_cen[3 * sa + 1], Az = shell_cen[3 * sa + 2];
-    const double Bx = shell_cen[3 * sb], By = shell_cen[3 * sb + 1], Bz = shell_cen[3 * sb + 2];
+    const double al = exps[exp_off[sa] + ia], be = exps[exp_off[sb] + ib];
+    const double ca = coefs[coef_off[sa] + ia], cb = coefs[coef_off[sb] + ib];
+    const int ca3 = 3 * cen_idx[sa], cb3 = 3 * cen_idx[sb];
+    const double Ax = shell_cen[ca3], Ay = shell_cen[ca3 + 1], Az = shell_cen[ca3 + 2];
+    const double Bx = shell_cen[cb3], By = shell_cen[cb3 + 1], Bz = shell_cen[cb3 + 2];
     const double ABx = Ax - Bx, ABy = Ay - By, ABz = Az - Bz;
     const double z = al + be, oz = 1.0 / z;
     const double r2 = ABx * ABx + ABy * ABy + ABz * ABz;
@@ -220,7 +225,10 @@ struct intc_plan {
     int *q_size = nullptr;        // integrals in the quartet's block
     int *pair_sa = nullptr, *pair_sb = nullptr, *pair_off = nullptr, *pair_np = nullptr, *pair_nk = nullptr;
     int *pp_pair = nullptr;
-    int *shell_off = nullptr;
+    int *shell_off = nullptr;     // per plan shell: exponent offset
+    int *coef_off = nullptr;      // per plan shell: coefficient offset
+    int *cen_idx = nullptr;       // per plan shell: centre index
+    bool strided = false;         // caller strides or contractions: blocks are not dense
     // routed queues: per bin, FP64, FP32 and skip index lists (segments at q0)
     // and their lengths (3 per bin)
     int *idx64 = nullptr, *idx32 = nullptr, *idxs = nullptr, *counts = nullptr;
@@ -276,7 +284,7 @@ void plan_free(intc_plan *p)
     p->events.clear();
     p->side.clear();
     void *ptrs[] = {p->q_bra, p->q_ket, p->q_off, p->q_str, p->q_orig, p->q_bin, p->q_size, p->pair_sa,
-                    p->pair_sb, p->pair_off, p->pair_np, p->pair_nk, p->pp_pair, p->shell_off, p->idx64,
+                    p->pair_sb, p->pair_off, p->pair_np, p->pair_nk, p->pp_pair, p->shell_off, p->coef_off, p->cen_idx, p->idx64,
                     p->idx32, p->idxs, p->counts, p->tile_q0, p->tile_n, p->tile_bin, p->bin_tile0,
                     p->tile_cnt, p->tile_base, p->bin_q0, p->pz, p->po2z, p->ppx, p->ppy, p->ppz, p->pkap,
                     p->pairA, p->pairAB, p->pair_np_scr};
@@ -534,7 +542,7 @@ struct NoAllocHead {
     double sph[intc_sph_off(8)];
     long long q_off[NA_COMBO];
     unsigned long long q_str[NA_COMBO];
-    int shell_off[NA_SHELL];
+    int shell_off[NA_SHELL], vid[NA_SHELL];
     int pair_sa[NA_PAIR], pair_sb[NA_PAIR], pair_off[NA_PAIR], pair_np[NA_PAIR], pair_nk[NA_PAIR];
     int pp_pair[NA_PP];
     int q_bra[NA_COMBO], q_ket[NA_COMBO];
@@ -577,7 +585,8 @@ static int finish_plan(intc_plan *p, const std::vector<const IntcClassEntry *> &
                        std::vector<int> &q_bra, std::vector<int> &q_ket, std::vector<long long> &q_off,
                        std::vector<unsigned long long> &q_str, std::vector<int> &q_size, std::vector<int> &q_orig,
                        const std::vector<int> &pair_sa, const std::vector<int> &pair_sb, const int *shell_nprim,
-                       const int *shell_off, int n_shell)
+                       const int *shell_off, int n_shell, const int *coef_off = nullptr,
+                       const int *cen_idx = nullptr)
 {
     const long long nq = (long long)q_bra.size();
     const size_t npair = pair_sa.size();
@@ -634,6 +643,9 @@ static int finish_plan(intc_plan *p, const std::vector<const IntcClassEntry *> &
     p->identity_order = true;
     for (long long i = 0; i < nq && p->identity_order; ++i) p->identity_order = q_orig[i] == i;
     std::vector<int> soff(shell_off, shell_off + n_shell);
+    std::vector<int> coff(coef_off ? coef_off : shell_off, (coef_off ? coef_off : shell_off) + n_shell);
+    std::vector<int> cidx(n_shell);
+    for (int s2 = 0; s2 < n_shell; ++s2) cidx[s2] = cen_idx ? cen_idx[s2] : s2;
     int rc = 0;
     rc |= dev_upload(&p->tile_q0, tile_q0);
     rc |= dev_upload(&p->tile_n, tile_n);
@@ -654,6 +666,8 @@ static int finish_plan(intc_plan *p, const std::vector<const IntcClassEntry *> &
     rc |= dev_upload(&p->pair_nk, pair_nk);
     rc |= dev_upload(&p->pp_pair, pp_pair);
     rc |= dev_upload(&p->shell_off, soff);
+    rc |= dev_upload(&p->coef_off, coff);
+    rc |= dev_upload(&p->cen_idx, cidx);
     if (rc) return rc;
     p->n_pair = (int)npair;
     p->n_pp = npp;
@@ -742,10 +756,16 @@ static bool vrr_on_second(int a, int b, const int *shell_nprim, const int *shell
     return wb < wa;
 }
 
-INTC_API int intc_plan_create(intc_plan **out, int spherical, int n_shell, const int *shell_l,
-                              const int *shell_nprim, const int *shell_off, const double *shell_cen,
-                              const double *exps, const double *coefs, long long n_quartet, const int *quartets,
-                              const long long *out_off)
+// Plan over plan shells (shell_l, shell_nprim; host exps/coefs at shell_off
+// for the orientation choice).  strides: 4 per quartet (caller order a, b,
+// c, d) or NULL for dense row-major blocks.  dev_exp_off / dev_coef_off /
+// dev_cen: where the pair builder finds each plan shell's exponents,
+// coefficients and centre in the device arrays execute receives (NULL: at
+// shell_off and the shell's own index).
+static int plan_create_impl(intc_plan **out, int spherical, int n_shell, const int *shell_l, const int *shell_nprim,
+                            const int *shell_off, const double *shell_cen, const double *exps, const double *coefs,
+                            long long n_quartet, const int *quartets, const long long *out_off, const int *strides,
+                            const int *dev_exp_off, const int *dev_coef_off, const int *dev_cen)
 {
     *out = nullptr;
     if (n_quartet < 0) return fail(-1, "negative size");
@@ -805,17 +825,101 @@ INTC_API int intc_plan_create(intc_plan **out, int spherical, int n_shell, const
         }
         q_size[i] = n[0] * n[1] * n[2] * n[3];
         int st[4] = {n[1] * n[2] * n[3], n[2] * n[3], n[3], 1};
+        if (strides)
+            for (int k = 0; k < 4; ++k) st[k] = strides[4 * (size_t)q + k];
         unsigned long long pk = 0;
         for (int k = 0; k < 4; ++k) pk |= (unsigned long long)(st[pm[k]] & 0xffff) << (16 * k);
         q_str[i] = pk;
         qcls[i] = cls[q];
         qprims[i] = prims[q];
     }
+    p->strided = strides != nullptr;
     rc = finish_plan(p, qcls, qprims, q_bra, q_ket, q_off, q_str, q_size, order, pair_sa, pair_sb, shell_nprim,
-                     shell_off, n_shell);
+                     dev_exp_off ? dev_exp_off : shell_off, n_shell, dev_coef_off, dev_cen);
     if (rc) { plan_free(p); delete p; return rc; }
     *out = p;
     return 0;
+}
+
+INTC_API int intc_plan_create(intc_plan **out, int spherical, int n_shell, const int *shell_l,
+                              const int *shell_nprim, const int *shell_off, const double *shell_cen,
+                              const double *exps, const double *coefs, long long n_quartet, const int *quartets,
+                              const long long *out_off)
+{
+    return plan_create_impl(out, spherical, n_shell, shell_l, shell_nprim, shell_off, shell_cen, exps, coefs,
+                            n_quartet, quartets, out_off, nullptr, nullptr, nullptr, nullptr);
+}
+
+// General contraction and caller strides: shell s has shell_ncont[s]
+// contractions, column k of its coefficients at coefs[coef_off[s] + k nprim];
+// quartet q writes element (ia, ib, ic, id) of contraction (ka, kb, kc, kd) at
+// out[out_off[q] + sum_X (iX iskip[4q+X] + kX iskip_cont[4q+X])], the
+// reference's per-quartet addressing (contracted_general.py:1808-1875).  Each
+// (shell, contraction) becomes one plan shell and each quartet one plan
+// quartet per combination of contractions.
+INTC_API int intc_plan_create_general(intc_plan **out, int spherical, int n_shell, const int *shell_l,
+                                      const int *shell_nprim, const int *shell_ncont, const int *shell_off,
+                                      const int *coef_off, const double *shell_cen, const double *exps,
+                                      const double *coefs, long long n_quartet, const int *quartets,
+                                      const long long *out_off, const int *iskip, const int *iskip_cont)
+{
+    *out = nullptr;
+    if (n_quartet < 0 || n_shell < 0) return fail(-1, "negative size");
+    if (!shell_ncont || !coef_off || !iskip || !iskip_cont) return fail(-1, "shell_ncont, coef_off, iskip and "
+                                                                           "iskip_cont are required");
+    std::vector<int> base(n_shell), v_l, v_np, v_eoff, v_coff, v_cen, v_hoff;
+    std::vector<double> h_ex, h_co, h_cen;
+    for (int s2 = 0; s2 < n_shell; ++s2) {
+        if (shell_ncont[s2] < 1) return fail(-1, "shell_ncont must be >= 1");
+        if (shell_nprim[s2] < 1) return fail(-1, "shell_nprim must be >= 1");
+        base[s2] = (int)v_l.size();
+        for (int k = 0; k < shell_ncont[s2]; ++k) {
+            v_l.push_back(shell_l[s2]);
+            v_np.push_back(shell_nprim[s2]);
+            v_eoff.push_back(shell_off[s2]);
+            v_coff.push_back(coef_off[s2] + k * shell_nprim[s2]);
+            v_cen.push_back(s2);
+            v_hoff.push_back((int)h_ex.size());
+            for (int i = 0; i < shell_nprim[s2]; ++i) {
+                h_ex.push_back(exps ? exps[shell_off[s2] + i] : 0.0);
+                h_co.push_back(coefs ? coefs[coef_off[s2] + k * shell_nprim[s2] + i] : 0.0);
+            }
+            for (int d = 0; d < 3; ++d) h_cen.push_back(shell_cen ? shell_cen[3 * s2 + d] : 0.0);
+        }
+    }
+    std::vector<int> vq;
+    std::vector<long long> voff;
+    std::vector<int> vst;
+    for (long long q = 0; q < n_quartet; ++q) {
+        const int *sq = quartets + 4 * q;
+        int nc[4];
+        for (int k = 0; k < 4; ++k) {
+            if (sq[k] < 0 || sq[k] >= n_shell) return fail(-1, "quartet shell id out of range");
+            nc[k] = shell_ncont[sq[k]];
+            const int st = iskip[4 * q + k];
+            if (st < 0 || st > 0xffff) return fail(-1, "iskip must be in [0, 65535] (16-bit strides)");
+            if (iskip_cont[4 * q + k] < 0) return fail(-1, "iskip_cont must be >= 0");
+        }
+        for (int ka = 0; ka < nc[0]; ++ka)
+            for (int kb = 0; kb < nc[1]; ++kb)
+                for (int kc = 0; kc < nc[2]; ++kc)
+                    for (int kd = 0; kd < nc[3]; ++kd) {
+                        const int kk[4] = {ka, kb, kc, kd};
+                        long long o = out_off[q];
+                        for (int k = 0; k < 4; ++k) {
+                            vq.push_back(base[sq[k]] + kk[k]);
+                            vst.push_back(iskip[4 * q + k]);
+                            o += (long long)kk[k] * iskip_cont[4 * q + k];
+                        }
+                        voff.push_back(o);
+                    }
+    }
+    const bool have_host = shell_cen && exps && coefs;
+    int rc = plan_create_impl(out, spherical, (int)v_l.size(), v_l.data(), v_np.data(), v_hoff.data(),
+                              have_host ? h_cen.data() : nullptr, have_host ? h_ex.data() : nullptr,
+                              have_host ? h_co.data() : nullptr, (long long)voff.size(), vq.data(), voff.data(),
+                              vst.data(), v_eoff.data(), v_coff.data(), v_cen.data());
+    return rc;
 }
 
 // Quartets given as pairs of shell pairs: pair P = (pair_shells[2P], pair_shells[2P+1]),
@@ -996,7 +1100,8 @@ INTC_API int intc_plan_build_pairs(intc_plan *p, const double *exps, const doubl
     hipStream_t st = (hipStream_t)stream;
     int blocks = (int)((p->n_pp + 255) / 256);
     hipLaunchKernelGGL(intc_build_pairs, dim3(blocks), dim3(256), 0, st, p->n_pp, p->pp_pair, p->pair_sa,
-                       p->pair_sb, p->pair_off, p->pair_nk, p->shell_off, shell_cen, exps, coefs, p->pz, p->po2z,
+                       p->pair_sb, p->pair_off, p->pair_nk, p->shell_off, p->coef_off, p->cen_idx, shell_cen, exps,
+                       coefs, p->pz, p->po2z,
                        p->ppx, p->ppy, p->ppz, p->pkap, p->pairA, p->pairAB);
     HIPCHK(hipGetLastError());
     if (p->prim_cut > 0.0) {
@@ -1353,6 +1458,8 @@ INTC_API int intc_plan_execute_routed(intc_plan *p, const double *exps, const do
                                       double *out, void *stream)
 {
     if (!route) return fail(-1, "route array is required");
+    if (p->strided)
+        return fail(-1, "routed execution needs dense output blocks (a plan without caller strides or contractions)");
     if (p->persistent) {
         int rc = prepare_multi(p, sph, out, true, p->m_eager);
         if (rc) return rc;
@@ -1371,6 +1478,8 @@ INTC_API int intc_plan_graph_create(intc_plan *p, const double *exps, const doub
                                     void *stream)
 {
     if (n_streams < 1 || n_streams > 16) return fail(-1, "n_streams must be in [1, 16]");
+    if (route && p->strided)
+        return fail(-1, "routed execution needs dense output blocks (a plan without caller strides or contractions)");
     if (p->graph_exec) {
         (void)hipGraphExecDestroy(p->graph_exec);
         p->graph_exec = nullptr;
@@ -1668,6 +1777,7 @@ static int per_quartet_noalloc(const ShellArg sa[4], double *output)
         base[k] = ns;
         for (int j = 0; j < sa[k].n_cont; ++j, ++ns) {
             h.shell_off[ns] = np;
+            h.vid[ns] = ns;
             for (int i = 0; i < 3; ++i) h.cen[3 * ns + i] = sa[k].c[i];
             for (int i = 0; i < sa[k].n_prim; ++i, ++np) {
                 h.exps[np] = sa[k].x[i];
@@ -1723,7 +1833,8 @@ static int per_quartet_noalloc(const ShellArg sa[4], double *output)
     double *dout = scr + NA_OUT_OFFSET;
     HIPCHK(hipMemcpy(dh, &h, sizeof(h), hipMemcpyHostToDevice));
     hipLaunchKernelGGL(intc_build_pairs, dim3((npp + 255) / 256), dim3(256), 0, (hipStream_t)0, (long long)npp,
-                       dh->pp_pair, dh->pair_sa, dh->pair_sb, dh->pair_off, dh->pair_nk, dh->shell_off, dh->cen,
+                       dh->pp_pair, dh->pair_sa, dh->pair_sb, dh->pair_off, dh->pair_nk, dh->shell_off, dh->shell_off,
+                       dh->vid, dh->cen,
                        dh->exps, dh->coefs, pz, po2z, ppx, ppy, ppz, pkap, pairA, pairAB);
     HIPCHK(hipGetLastError());
     IntcEriArgs a;

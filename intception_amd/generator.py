@@ -44,21 +44,29 @@ class parallelism:
 
 class generator_options:
     """Reference options (contracted, spherical_transformed, global_user_lmax,
-    use_cblas, output_directory, suffixes, silent) plus this build's:
-    backend ("hip": bind to libintc's kernels; "c99": emit CPU C99 source of the
-    ERI class), precision ("fp64" or "mixed"), boys ("chebyshev" or
-    "reference"), screening threshold."""
+    use_cblas, output_directory, suffixes, silent; generator.py:102-334) plus
+    this build's: backend ("hip": bind to libintc's kernels; "c99": emit the
+    C99 file set of the ERI class) and boys ("chebyshev" or "reference", the
+    device Boys scheme).  use_cblas=True is refused: the reference's CBLAS
+    contraction path does not run (an undefined name,
+    generator_contract_function.py:264) and this build has no CBLAS path.
+    Mixed precision and screening are properties of a run, not of the
+    generated code: they come with the plan (EriPlan.run_routed with a
+    screening.LearnedRouter, EriPlan.set_prim_screen)."""
 
     def __init__(self, contracted=None, spherical_transformed=None, global_user_lmax=None, use_cblas=None,
                  output_directory=None, auto_suffix=None, prim_suffix=None, cont_suffix=None, cart_suffix=None,
-                 sph_suffix=None, silent=None, backend=None, precision=None, boys=None, screening=None,
+                 sph_suffix=None, silent=None, backend=None, boys=None,
                  **unused):
         self._contracted = True if contracted is None else bool(contracted)
         self._sph = True if spherical_transformed is None else bool(spherical_transformed)
         if self._sph and not self._contracted:
             raise ValueError("spherical transformation requires contracted integrals")
         self._lmax = 6 if global_user_lmax is None else int(global_user_lmax)
-        self._cblas = bool(use_cblas)
+        if use_cblas:
+            raise NotImplementedError("use_cblas=True: the reference's CBLAS contraction does not run "
+                                      "(generator_contract_function.py:264) and this build has no CBLAS path")
+        self._cblas = False
         self._outdir = output_directory or "./output"
         self._auto_suffix = True if auto_suffix is None else auto_suffix
         self._suffix = {"prim": prim_suffix or "prim", "cont": cont_suffix or "cont", "cart": cart_suffix or "cart",
@@ -67,11 +75,18 @@ class generator_options:
         self._backend = backend or "hip"
         if self._backend not in ("hip", "c99"):
             raise ValueError("this build's generator has the 'hip' and 'c99' backends")
-        self._precision = precision or "fp64"
-        assert self._precision in ("fp64", "mixed")
         self._boys = boys or "chebyshev"
-        assert self._boys in ("chebyshev", "reference")
-        self._screening = 1e-12 if screening is None else float(screening)
+        if self._boys not in ("chebyshev", "reference"):
+            raise ValueError("boys must be 'chebyshev' or 'reference'")
+        # the reference's remaining options are accepted; they do not change
+        # this build's output (file names are fixed, no CBLAS, no
+        # vectorisation placeholder); anything else is a mistake
+        known = {"spherical_transform_conditionals", "pllm", "default_boys_f", "global_data_filename",
+                 "global_functions_filename", "boys_function_filename", "intception_main_filename",
+                 "cblas_header_filename", "cblas_dgemm_min_lengths", "add_suffix_to_global", "stdout_width"}
+        bad = sorted(set(unused) - known)
+        if bad:
+            raise TypeError("generator_options: unknown option(s) %s" % ", ".join(bad))
         self._unused = unused
 
     def contracted(self): return self._contracted
@@ -80,7 +95,6 @@ class generator_options:
     def use_cblas(self): return self._cblas
     def output_directory(self): return self._outdir
     def backend(self): return self._backend
-    def precision(self): return self._precision
     def boys(self): return self._boys
     def parallelism(self): return parallelism()
 
@@ -548,7 +562,7 @@ class generator:
                 fh.write(b.header())
             manifest[n] = {"integral": b.integral.name(), "kind": getattr(b, "kind", "eri"),
                            "spherical": b.spherical, "backend": "hip",
-                           "precision": self._opt.precision(), "boys": self._opt.boys(),
+                           "boys": self._opt.boys(),
                            "library": "libintc.so", "header": "intc.h"}
         with open(os.path.join(d, "intception_hip_manifest.json"), "w") as fh:
             json.dump(manifest, fh, indent=1)

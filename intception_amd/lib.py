@@ -34,6 +34,8 @@ SIGNATURES = [
                                  _llp]),
     ("intc_plan_create_pairs", c_int, [ctypes.POINTER(c_void_p), c_int, c_int, _ip, _ip, _ip, _dp, _dp, _dp, c_int,
                                        _ip, c_ll, _ip, _llp]),
+    ("intc_plan_create_general", c_int, [ctypes.POINTER(c_void_p), c_int, c_int, _ip, _ip, _ip, _ip, _ip, _dp, _dp,
+                                         _dp, c_ll, _ip, _llp, _ip, _ip]),
     ("intc_plan_num_bins", c_int, [c_void_p]),
     ("intc_plan_bin_info", c_int, [c_void_p, c_int, c_int, _ip, _ip, _ip, _ip, _ip]),
     ("intc_plan_set_bin_tuning", c_int, [c_void_p, c_int, c_int, c_int, c_int]),

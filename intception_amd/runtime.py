@@ -228,6 +228,56 @@ class EriPlan:
             pass
 
 
+class GeneralEriPlan(EriPlan):
+    """A plan over general contractions with the caller's strides (the
+    per-quartet fields of the reference's <full>_main for a batch).  shells:
+    list of dicts with l, exps (n_prim), coefs (n_prim x n_cont, normalised)
+    and cen; quartets (n, 4) shell indices; out_off[q], iskip[q] (4) and
+    iskip_cont[q] (4) place element (ia, ib, ic, id) of contraction
+    (ka, kb, kc, kd) at out[out_off[q] + sum_X(iX iskip_X + kX iskip_cont_X)].
+    out_size: length of the output array.  run(), build_pairs(), run_bin()
+    and capture() as for EriPlan; routed execution is refused."""
+
+    def __init__(self, spherical, shells, quartets, out_off, iskip, iskip_cont, out_size, device="cuda", sph=None):
+        torch = _torch()
+        L = _lib.load()
+        self._L = L
+        self.batch = None
+        self.device = _cuda_device(torch, device)
+        ls = np.array([s["l"] for s in shells], dtype=np.int32)
+        cols = [np.asarray(s["coefs"], dtype=np.float64).reshape(len(s["exps"]), -1) for s in shells]
+        nprim = np.array([len(s["exps"]) for s in shells], dtype=np.int32)
+        ncont = np.array([c.shape[1] for c in cols], dtype=np.int32)
+        soff = np.concatenate([[0], np.cumsum(nprim)[:-1]]).astype(np.int32)
+        coff = np.concatenate([[0], np.cumsum(nprim * ncont)[:-1]]).astype(np.int32)
+        ex = np.ascontiguousarray(np.concatenate([np.asarray(s["exps"], dtype=np.float64) for s in shells]))
+        co = np.ascontiguousarray(np.concatenate([c.reshape(-1, order="F") for c in cols]))
+        cen = np.ascontiguousarray(np.array([s["cen"] for s in shells], dtype=np.float64).reshape(-1))
+        q = np.ascontiguousarray(np.asarray(quartets, dtype=np.int32).reshape(-1))
+        oo = np.ascontiguousarray(np.asarray(out_off, dtype=np.int64))
+        isk = np.ascontiguousarray(np.asarray(iskip, dtype=np.int32).reshape(-1))
+        iskc = np.ascontiguousarray(np.asarray(iskip_cont, dtype=np.int32).reshape(-1))
+        ip = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+        dp = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        h = ctypes.c_void_p()
+        with torch.cuda.device(self.device):
+            _lib.check(L.intc_plan_create_general(ctypes.byref(h), int(bool(spherical)), len(ls), ip(ls), ip(nprim),
+                                                  ip(ncont), ip(soff), ip(coff), dp(cen), dp(ex), dp(co), len(oo),
+                                                  ip(q), oo.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong)),
+                                                  ip(isk), ip(iskc)))
+        self.handle = h
+        self.d_exps = torch.from_numpy(ex).to(self.device)
+        self.d_coefs = torch.from_numpy(co).to(self.device)
+        self.d_cen = torch.from_numpy(cen).to(self.device)
+        self.d_sph = None
+        if spherical and sph is not None:
+            self.d_sph = torch.from_numpy(np.ascontiguousarray(sph, dtype=np.float64)).to(self.device)
+        self.out_size = int(out_size)
+
+    def run_routed(self, route, out=None):
+        raise NotImplementedError("routed execution needs dense output blocks")
+
+
 class OneEPlan:
     """Batched one-electron integrals: kind 0 overlap (items = shell pairs
     (a, b)), kind 1 nuclear attraction (items = (a, b, c), c indexing

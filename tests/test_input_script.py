@@ -158,3 +158,14 @@ def test_onee_scripts_recognised(tmp_path):
     assert "#define intception_integral_nuclear_2idx_ab_m_cc_prim_cart_main intc_nuclear_prim_cart_main" in hdr
     assert json.load(open(tmp_path / "intception_hip_manifest.json"))[
         "intception_integral_overlap_2idx_ab_prim_cart"]["kind"] == "overlap"
+
+
+def test_generator_options_refuse_what_they_cannot_do():
+    """use_cblas=True (broken upstream, no CBLAS path here) and unknown
+    options raise instead of being ignored; the reference's other options are
+    accepted."""
+    with pytest.raises(NotImplementedError, match="CBLAS"):
+        generator_options(use_cblas=True)
+    with pytest.raises(TypeError, match="precision"):
+        generator_options(precision="mixed")
+    generator_options(stdout_width=100, add_suffix_to_global=True, silent=True)
