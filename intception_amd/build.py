@@ -124,7 +124,7 @@ def build(lmax=2, jobs=None, verbose=True, force=False):
     srcs.append((os.path.join(CSRC, "intc_runtime.hip"),
                  common + [inc, os.path.join(GEN, "class_table.inc")]))
     srcs.append((os.path.join(CSRC, "intc_mlp.hip"), []))
-    srcs.append((os.path.join(CSRC, "intc_eri_wg.hip"), common))
+    srcs.append((os.path.join(CSRC, "intc_eri_wg.hip"), common + [os.path.join(GEN, "wg_classes.inc")]))
     srcs.append((os.path.join(CSRC, "intc_onee.hip"), common + [os.path.join(GEN, "onee_kernels.inc")]))
     jobs = jobs or min(16, os.cpu_count() or 4)
     with ThreadPoolExecutor(jobs) as ex:

@@ -1025,6 +1025,14 @@ def generate(outdir, classes, sph_modes=(False, True), precs=("d", "f"), wg=(), 
     for i in range(0, len(blob), 4):
         lines.append("    " + ", ".join(repr(float(v)) for v in blob[i:i + 4]) + ",")
     lines.append("};")
+    # the workgroup kernel's class list (its template instantiations)
+    wsrc = "// Generated by intception_amd/codegen.py -- do not edit.\n" + "".join(
+        "INTC_WG_CLASS(%d, %d, %d, %d)\n" % tuple(c) for c in wg)
+    fn = os.path.join(outdir, "wg_classes.inc")
+    old = open(fn).read() if os.path.exists(fn) else None
+    if old != wsrc:
+        with open(fn, "w") as fh:
+            fh.write(wsrc)
     tab = os.path.join(outdir, "class_table.inc")
     src = "\n".join(lines) + "\n"
     old = open(tab).read() if os.path.exists(tab) else None

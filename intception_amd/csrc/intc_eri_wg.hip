@@ -28,6 +28,8 @@
 // VT = double: plain FP64.  VT = float: the VRR tables in FP32 on s^m-scaled
 // values (s = max(1, 2T); every m+1 term carries 1/s), Boys, prefactors and
 // the contraction sums in FP64, HRR and transforms in FP64.
+#include <utility>
+
 #include "intc_common.h"
 
 #define INTC_API extern "C" __attribute__((visibility("default")))
@@ -167,38 +169,170 @@ __device__ double *wg_hrr(const WgTab &tb, int R, int lo, int nmove, const doubl
     return prev;
 }
 
-template <int BM>
-__device__ void wg_boys(int M, double T, double *F, const IntcEriArgs &a)
+
+// Compile-time layout of the recurrences of canonical class (LA LB | LC LD)
+// (every size below is a constant of the kernel instantiation):
+//   ket level kl stores (e|f)^(m) for |f| = kl, m <= Lk - kl and |e| >= elev(kl),
+//   the lowest |e| a later level or the accumulation can read:
+//   elev(kl) = max(0, min(LA, LB) - (Lk - kl)) (either orientation of the pair);
+//   layout [e - e0(kl)][f - ncc0(kl - 1)][m]; level 0 is (e|0)^(m), m <= Lk.
+//   Work items of level kl are its (e, f) pairs, item j = e_rel * nf + f_rel,
+//   each computing its m-vector; thread t takes items t + 256 r.
+template <int LA, int LB, int LC, int LD>
+struct Wg2 {
+    static constexpr int L = LA + LB, Lk = LC + LD, M = L + Lk;
+    static constexpr int nAll = ncc0(L);
+    static constexpr int lminb = LA < LB ? LA : LB, lmink = LC < LD ? LC : LD;
+    static constexpr int elev(int kl) { return lminb - (Lk - kl) > 0 ? lminb - (Lk - kl) : 0; }
+    static constexpr int e0(int kl) { return ncc0(elev(kl) - 1); }
+    static constexpr int ne(int kl) { return nAll - e0(kl); }
+    static constexpr int nf(int kl) { return ncc(kl); }
+    static constexpr int nm(int kl) { return Lk - kl + 1; }
+    static constexpr int items(int kl) { return ne(kl) * nf(kl); }
+    static constexpr int size(int kl) { return items(kl) * nm(kl); }
+    static constexpr int reps(int kl) { return (items(kl) + WG_BLOCK - 1) / WG_BLOCK; }
+    static constexpr int cap()
+    {
+        int c = 0;
+        for (int kl = 0; kl <= Lk; ++kl) c = size(kl) > c ? size(kl) : c;
+        return c;
+    }
+    // accumulator slots of levels >= max(1, lmink), in level order
+    static constexpr int acc_lo = lmink > 1 ? lmink : 1;
+    static constexpr int acc_base(int kl)
+    {
+        int b = 0;
+        for (int k2 = acc_lo; k2 < kl; ++k2) b += reps(k2);
+        return b;
+    }
+    static constexpr int nacc() { return acc_base(Lk + 1) > 0 ? acc_base(Lk + 1) : 1; }
+    // setup slot: c0_m (m <= M), PA, WP, QC, WQ (3 each), roz, roe, o2s, o2z, o2e, pref
+    static constexpr int NSLOT = M + 1 + 18;
+    static constexpr int CH = 64;   // primitive quartets per setup chunk
+};
+
+// per primitive quartet values the levels read, converted once per quartet
+template <typename VT>
+struct Wg2Prim {
+    VT QC[3], WQ[3], roe, o2e, o2s;
+    double pref;
+};
+
+// One ket level: items j = threadIdx.x + 256 r (r unrolled) compute their
+// m-vector from levels KL-1 and KL-2 (same e) and level KL-1 at e - 1_i:
+//   (e|f+1_i)^(m) = QC_i (e|f)^(m) + WQ_i (e|f)^(m+1)
+//                 + f_i/(2 eta) [(e|f-1_i)^(m) - rho/eta (e|f-1_i)^(m+1)]
+//                 + e_i/(2(zeta+eta)) (e-1_i|f)^(m+1)
+// and, from level max(1, lmin_k) on, add m = 0 to the thread's accumulators
+// (only items with la' <= |e| and lc' <= KL count for the oriented class).
+template <class W, int KL, typename VT>
+__device__ __forceinline__ void wg2_level(const WgTab &tb, VT *const lv[3], const Wg2Prim<VT> &pq, double *acc,
+                                          int e0o, int lco)
 {
-    switch (M) {
-#define INTC_WG_BOYS(m) \
-    case m: intc_boys<m, BM>(T, F, a); break;
-        INTC_WG_BOYS(0) INTC_WG_BOYS(1) INTC_WG_BOYS(2) INTC_WG_BOYS(3) INTC_WG_BOYS(4) INTC_WG_BOYS(5)
-        INTC_WG_BOYS(6) INTC_WG_BOYS(7) INTC_WG_BOYS(8) INTC_WG_BOYS(9) INTC_WG_BOYS(10) INTC_WG_BOYS(11)
-        INTC_WG_BOYS(12)
-#undef INTC_WG_BOYS
-        default: break;
+    constexpr int NF = W::nf(KL), NM = W::nm(KL);
+    constexpr int NF1 = W::nf(KL - 1), NM1 = W::nm(KL - 1);
+    constexpr int NF2 = KL >= 2 ? W::nf(KL - 2) : 1, NM2 = KL >= 2 ? W::nm(KL - 2) : 1;
+    constexpr int E0 = W::e0(KL), E01 = W::e0(KL - 1), E02 = KL >= 2 ? W::e0(KL - 2) : 0;
+    constexpr int G0 = ncc0(KL - 1), F10 = ncc0(KL - 2), F20 = ncc0(KL - 3);
+    VT *Kc = lv[KL % 3];
+    const VT *K1 = lv[(KL - 1) % 3];
+    const VT *K2 = lv[(KL + 1) % 3];   // level KL - 2
+#pragma unroll
+    for (int r = 0; r < W::reps(KL); ++r) {
+        const int j = threadIdx.x + WG_BLOCK * r;
+        if (j < W::items(KL)) {
+            const int ei = j / NF, fi = j - ei * NF;
+            const int ef = E0 + ei, gf = G0 + fi;
+            const int i = tb.dir[gf];
+            const int f1 = tb.down[3 * gf + i];
+            const int cf = tb.xyz[3 * gf + i] - 1;
+            const int ce = tb.xyz[3 * ef + i];
+            const VT *R1 = K1 + ((ef - E01) * NF1 + (f1 - F10)) * NM1;
+            const VT qc = i == 0 ? pq.QC[0] : (i == 1 ? pq.QC[1] : pq.QC[2]);
+            const VT wq = i == 0 ? pq.WQ[0] : (i == 1 ? pq.WQ[1] : pq.WQ[2]);
+            VT v[NM];
+#pragma unroll
+            for (int m = 0; m < NM; ++m) v[m] = qc * R1[m] + wq * R1[m + 1];
+            if (KL >= 2 && cf > 0) {
+                const int f2 = tb.down[3 * f1 + i];
+                const VT *R2 = K2 + ((ef - E02) * NF2 + (f2 - F20)) * NM2;
+                const VT c = (VT)cf * pq.o2e;
+#pragma unroll
+                for (int m = 0; m < NM; ++m) v[m] += c * (R2[m] - pq.roe * R2[m + 1]);
+            }
+            if (ce > 0) {
+                const int em = tb.down[3 * ef + i];
+                const VT *R3 = K1 + ((em - E01) * NF1 + (f1 - F10)) * NM1 + 1;
+                const VT c = (VT)ce * pq.o2s;
+#pragma unroll
+                for (int m = 0; m < NM; ++m) v[m] += c * R3[m];
+            }
+            VT *o = Kc + j * NM;
+#pragma unroll
+            for (int m = 0; m < NM; ++m) o[m] = v[m];
+            if constexpr (KL >= W::acc_lo) {
+                if (KL >= lco && ef >= e0o) {
+                    if (sizeof(VT) == 4)
+                        acc[W::acc_base(KL) + r] += pq.pref * (double)v[0];
+                    else
+                        acc[W::acc_base(KL) + r] += (double)v[0];
+                }
+            }
+        }
+    }
+    __syncthreads();
+}
+
+template <class W, typename VT, int... KL>
+__device__ __forceinline__ void wg2_levels(std::integer_sequence<int, KL...>, const WgTab &tb, VT *const lv[3],
+                                           const Wg2Prim<VT> &pq, double *acc, int e0o, int lco)
+{
+    (wg2_level<W, KL + 1, VT>(tb, lv, pq, acc, e0o, lco), ...);
+}
+
+// the accumulated (e|f) of this thread's items into X [e - e0o][f - f0o]
+template <class W, int KL>
+__device__ __forceinline__ void wg2_store_level(const double *acc, double *Xb, int e0o, int f0o, int nFo, int lco)
+{
+    if constexpr (KL >= W::acc_lo) {
+        constexpr int NF = W::nf(KL), E0 = W::e0(KL), G0 = ncc0(KL - 1);
+#pragma unroll
+        for (int r = 0; r < W::reps(KL); ++r) {
+            const int j = threadIdx.x + WG_BLOCK * r;
+            if (j < W::items(KL)) {
+                const int ei = j / NF, fi = j - ei * NF;
+                const int ef = E0 + ei, gf = G0 + fi;
+                if (KL >= lco && ef >= e0o) Xb[(ef - e0o) * nFo + (gf - f0o)] = acc[W::acc_base(KL) + r];
+            }
+        }
     }
 }
 
-template <typename VT, int BM>
+template <class W, int... KL>
+__device__ __forceinline__ void wg2_store(std::integer_sequence<int, KL...>, const double *acc, double *Xb, int e0o,
+                                          int f0o, int nFo, int lco)
+{
+    (wg2_store_level<W, KL + 1>(acc, Xb, e0o, f0o, nFo, lco), ...);
+}
+
+template <typename VT, int BM, int LA, int LB, int LC, int LD>
 __global__ void __launch_bounds__(WG_BLOCK) intc_eri_wg_k(const IntcEriArgs a, const IntcWgCls k)
 {
+    using W = Wg2<LA, LB, LC, LD>;
     extern __shared__ __attribute__((aligned(16))) unsigned char wg_lds[];
     int *itab = (int *)(wg_lds + k.o_tab);
     wg_tables(itab);
     WgTab tb{itab, itab + 3 * WG_NCOMP, itab + 4 * WG_NCOMP, itab + 7 * WG_NCOMP};
-    VT *bra = (VT *)(wg_lds + k.o_bra);
-    VT *lvl[3] = {(VT *)(wg_lds + k.o_lvl), (VT *)(wg_lds + k.o_lvl) + k.lvl_cap,
-                  (VT *)(wg_lds + k.o_lvl) + 2 * k.lvl_cap};
+    double *slots = (double *)(wg_lds + k.o_bra);
+    VT *const lv[3] = {(VT *)(wg_lds + k.o_lvl), (VT *)(wg_lds + k.o_lvl) + W::cap(),
+                       (VT *)(wg_lds + k.o_lvl) + 2 * W::cap()};
     double *Zb = (double *)(wg_lds + k.o_Z);
     double *Xb = (double *)(wg_lds + k.o_X);
     double *hk = (double *)(wg_lds + k.o_hk);
     double *hb = (double *)(wg_lds + k.o_hb);
     __syncthreads();
 
-    const int L = k.L, Lk = k.Lk, M = k.M;
-    int aoff[WG_NR], alev[WG_NR];
+    constexpr int L = W::L, Lk = W::Lk;
 
     const int nq = a.nq_dev ? *a.nq_dev : a.nq;
     for (int qq = blockIdx.x; qq < nq; qq += gridDim.x) {
@@ -247,143 +381,108 @@ __global__ void __launch_bounds__(WG_BLOCK) intc_eri_wg_k(const IntcEriArgs a, c
         const int lc = fk ? k.ld : k.lc, ld = fk ? k.lc : k.ld;
         const int e0 = ncc0(la - 1), nE = ncc0(L) - e0;
         const int f0 = ncc0(lc - 1), nF = ncc0(Lk) - f0;
-        const int nEF = nE * nF;
-        // this thread's accumulators: entry j = threadIdx.x + 256 r of the E x F
-        // table; its ket level and offset (m = 0) within that level's buffer
-        for (int r = 0; r < WG_NR; ++r) {
-            const int j = threadIdx.x + WG_BLOCK * r;
-            alev[r] = -1;
-            aoff[r] = 0;
-            if (j < nEF) {
-                const int ei = j / nF, fj = j - ei * nF;
-                const int ef = e0 + ei, ff = f0 + fj;
-                int kl = 0;
-                while (ncc0(kl) <= ff) ++kl;
-                const int fi = ff - ncc0(kl - 1);
-                alev[r] = kl;
-                aoff[r] = (ef * ncc(kl) + fi) * (Lk - kl + 1);
-            }
-        }
         const double Ax = Vb[0], Ay = Vb[1], Az = Vb[2];
         const double Cx = Vk[0], Cy = Vk[1], Cz = Vk[2];
-        double acc[WG_NR];
-        for (int r = 0; r < WG_NR; ++r) acc[r] = 0.0;
-
-        for (int p = 0; p < Pb * Pk; ++p) {
-            const int ib = p / Pk, ik = p - ib * Pk;
-            const double zeta = a.pp_zeta[ob + ib], o2z = a.pp_o2z[ob + ib];
-            const double Px = a.pp_px[ob + ib], Py = a.pp_py[ob + ib], Pz = a.pp_pz[ob + ib];
-            const double kab = a.pp_kappa[ob + ib];
-            const double eta = a.pp_zeta[okk + ik], o2e = a.pp_o2z[okk + ik];
-            const double Qx = a.pp_px[okk + ik], Qy = a.pp_py[okk + ik], Qz = a.pp_pz[okk + ik];
-            const double kcd = a.pp_kappa[okk + ik];
-            const double os = 1.0 / (zeta + eta);
-            const double rho = zeta * eta * os;
-            const double PQx = Px - Qx, PQy = Py - Qy, PQz = Pz - Qz;
-            const double Tb = rho * (PQx * PQx + PQy * PQy + PQz * PQz);
-            const double pref = kab * kcd * sqrt(os);
-            const bool f32 = sizeof(VT) == 4;
-            const double sc = (f32 && Tb > 0.5) ? 2.0 * Tb : 1.0, isc = 1.0 / sc;
-            const double eo = eta * os, zo = zeta * os;
-            const VT PA[3] = {(VT)(Px - Ax), (VT)(Py - Ay), (VT)(Pz - Az)};
-            const VT WP[3] = {(VT)(-eo * PQx * isc), (VT)(-eo * PQy * isc), (VT)(-eo * PQz * isc)};
-            const VT QC[3] = {(VT)(Qx - Cx), (VT)(Qy - Cy), (VT)(Qz - Cz)};
-            const VT WQ[3] = {(VT)(zo * PQx * isc), (VT)(zo * PQy * isc), (VT)(zo * PQz * isc)};
-            const VT roz = (VT)(eo * isc), roe = (VT)(zo * isc), o2s = (VT)(0.5 * os * isc);
-            const VT o2zv = (VT)o2z, o2ev = (VT)o2e;
-            // the previous primitive's last readers of bra / level buffers are done
-            __syncthreads();
-            if (threadIdx.x == 0) {
-                double F[13];
-                wg_boys<BM>(M, Tb, F, a);
+        double acc[W::nacc()];
+#pragma unroll
+        for (int r = 0; r < W::nacc(); ++r) acc[r] = 0.0;
+        double acc0 = 0.0;   // level 0 (an s-shell ket VRR index), by the chain thread of e
+        // this thread's bra component for the chains: e = flat index threadIdx.x
+        const int te = threadIdx.x;
+        int cx = 0, cy = 0, cz = 0;
+        if (te < W::nAll) { cx = tb.xyz[3 * te]; cy = tb.xyz[3 * te + 1]; cz = tb.xyz[3 * te + 2]; }
+        const int P = Pb * Pk;
+        for (int c0p = 0; c0p < P; c0p += W::CH) {
+            const int np = min(W::CH, P - c0p);
+            __syncthreads();   // the previous chunk's slots are read
+            // setup of every primitive quartet of the chunk in parallel: Boys,
+            // prefactor, recurrence coefficients (VRR centres: the oriented A', C')
+            for (int t = threadIdx.x; t < np; t += WG_BLOCK) {
+                const int pq = c0p + t;
+                const int ib = pq / Pk, ik = pq - ib * Pk;
+                const double zeta = a.pp_zeta[ob + ib], o2z = a.pp_o2z[ob + ib];
+                const double Px = a.pp_px[ob + ib], Py = a.pp_py[ob + ib], Pz = a.pp_pz[ob + ib];
+                const double kab = a.pp_kappa[ob + ib];
+                const double eta = a.pp_zeta[okk + ik], o2e = a.pp_o2z[okk + ik];
+                const double Qx = a.pp_px[okk + ik], Qy = a.pp_py[okk + ik], Qz = a.pp_pz[okk + ik];
+                const double kcd = a.pp_kappa[okk + ik];
+                const double os = 1.0 / (zeta + eta);
+                const double rho = zeta * eta * os;
+                const double PQx = Px - Qx, PQy = Py - Qy, PQz = Pz - Qz;
+                const double Tb = rho * (PQx * PQx + PQy * PQy + PQz * PQz);
+                const double pref = kab * kcd * sqrt(os);
+                const bool f32 = sizeof(VT) == 4;
+                const double sc = (f32 && Tb > 0.5) ? 2.0 * Tb : 1.0, isc = 1.0 / sc;
+                const double eo = eta * os, zo = zeta * os;
+                double F[W::M + 1];
+                intc_boys<W::M, BM>(Tb, F, a);
+                double *u = slots + (size_t)t * W::NSLOT;
                 double s = 1.0;
-                for (int m = 0; m <= M; ++m) {
-                    bra[m] = f32 ? (VT)(F[m] * s) : (VT)(pref * F[m]);
+#pragma unroll
+                for (int m = 0; m <= W::M; ++m) {
+                    u[m] = f32 ? F[m] * s : pref * F[m];
                     s *= sc;
                 }
+                double *w = u + W::M + 1;
+                w[0] = Px - Ax; w[1] = Py - Ay; w[2] = Pz - Az;
+                w[3] = -eo * PQx * isc; w[4] = -eo * PQy * isc; w[5] = -eo * PQz * isc;
+                w[6] = Qx - Cx; w[7] = Qy - Cy; w[8] = Qz - Cz;
+                w[9] = zo * PQx * isc; w[10] = zo * PQy * isc; w[11] = zo * PQz * isc;
+                w[12] = eo * isc; w[13] = zo * isc; w[14] = 0.5 * os * isc; w[15] = o2z; w[16] = o2e;
+                // the prefactor the FP32 route multiplies at accumulation, after the 17 entries
+                w[17] = pref;
             }
             __syncthreads();
-            // bra levels: (e|0)^(m), e at level n, m <= M - n
-            for (int n = 1; n <= L; ++n) {
-                const int nm = M - n + 1, cnt = ncc(n) * nm, e0n = ncc0(n - 1);
-                const int nm1 = M - n + 2, nm2 = M - n + 3;
-                for (int t = threadIdx.x; t < cnt; t += WG_BLOCK) {
-                    const int ei = t / nm, m = t - ei * nm;
-                    const int ef = e0n + ei, i = tb.dir[ef];
-                    const int e1 = tb.down[3 * ef + i];
-                    const VT *B1 = bra + k.braoff[n - 1] + (e1 - ncc0(n - 2)) * nm1;
-                    VT v = PA[i] * B1[m] + WP[i] * B1[m + 1];
-                    const int ci = tb.xyz[3 * ef + i] - 1;
-                    if (ci > 0) {
-                        const int e2 = tb.down[3 * e1 + i];
-                        const VT *B2 = bra + k.braoff[n - 2] + (e2 - ncc0(n - 3)) * nm2;
-                        v += (VT)ci * o2zv * (B2[m] - roz * B2[m + 1]);
+            for (int pl = 0; pl < np; ++pl) {
+                const double *u = slots + (size_t)pl * W::NSLOT;
+                const double *w = u + W::M + 1;
+                Wg2Prim<VT> pq;
+#pragma unroll
+                for (int d = 0; d < 3; ++d) { pq.QC[d] = (VT)w[6 + d]; pq.WQ[d] = (VT)w[9 + d]; }
+                pq.roe = (VT)w[13];
+                pq.o2s = (VT)w[14];
+                pq.o2e = (VT)w[16];
+                pq.pref = w[17];
+                // bra chains: thread e builds (e|0)^(m), m <= M - |e|, along the
+                // direction rule's chain (z steps, then y, then x), in registers
+                if (te < W::nAll) {
+                    VT cm2[W::M + 1], cm1[W::M + 1], cc[W::M + 1];
+#pragma unroll
+                    for (int m = 0; m <= W::M; ++m) { cm1[m] = (VT)u[m]; cm2[m] = (VT)0; }
+                    const VT o2zv = (VT)w[15], roz = (VT)w[12];
+                    int s = 0;
+                    for (int d = 2; d >= 0; --d) {
+                        const int nd = d == 0 ? cx : (d == 1 ? cy : cz);
+                        const VT pa = (VT)(d == 0 ? w[0] : (d == 1 ? w[1] : w[2]));
+                        const VT wp = (VT)(d == 0 ? w[3] : (d == 1 ? w[4] : w[5]));
+                        for (int k2 = 0; k2 < nd; ++k2) {
+                            ++s;
+                            const VT kk = (VT)k2 * o2zv;
+#pragma unroll
+                            for (int m = 0; m < W::M; ++m)
+                                cc[m] = pa * cm1[m] + wp * cm1[m + 1] + kk * (cm2[m] - roz * cm2[m + 1]);
+                            cc[W::M] = (VT)0;
+#pragma unroll
+                            for (int m = 0; m <= W::M; ++m) { cm2[m] = cm1[m]; cm1[m] = cc[m]; }
+                        }
                     }
-                    bra[k.braoff[n] + ei * nm + m] = v;
+                    (void)s;
+                    if (te >= W::e0(0)) {
+                        VT *o = lv[0] + (te - W::e0(0)) * (W::Lk + 1);
+#pragma unroll
+                        for (int m = 0; m <= W::Lk; ++m) o[m] = cm1[m];
+                    }
+                    if (lc == 0 && te >= e0) acc0 += sizeof(VT) == 4 ? pq.pref * (double)cm1[0] : (double)cm1[0];
                 }
                 __syncthreads();
-            }
-            // ket level 0: (e|0)^(m) for every e with |e| <= L, m <= Lk
-            {
-                const int nm = Lk + 1, cnt = k.nAll * nm;
-                VT *K0 = lvl[0];
-                for (int t = threadIdx.x; t < cnt; t += WG_BLOCK) {
-                    const int ef = t / nm, m = t - ef * nm;
-                    int n = 0;
-                    while (ncc0(n) <= ef) ++n;
-                    K0[t] = bra[k.braoff[n] + (ef - ncc0(n - 1)) * (M - n + 1) + m];
-                }
-                __syncthreads();
-            }
-            // accumulate level 0 when the VRR shell of the ket is an s shell
-            // (an (s, l) ket pair oriented with the VRR on the s shell, or (ss))
-            if (lc == 0) {
-                for (int r = 0; r < WG_NR; ++r)
-                    if (alev[r] == 0) acc[r] += f32 ? pref * (double)lvl[0][aoff[r]] : (double)lvl[0][aoff[r]];
-            }
-            // ket levels
-            for (int kl = 1; kl <= Lk; ++kl) {
-                VT *Kc = lvl[kl % 3];
-                const VT *K1 = lvl[(kl - 1) % 3];
-                const VT *K2 = lvl[(kl + 1) % 3];  // level kl - 2
-                const int nm = Lk - kl + 1, nm1 = nm + 1, nm2 = nm + 2;
-                const int nf = ncc(kl), nf1 = ncc(kl - 1), nf2 = ncc(kl - 2);
-                const int g0 = ncc0(kl - 1), f10 = ncc0(kl - 2), f20 = ncc0(kl - 3);
-                const int cnt = k.nAll * nf * nm;
-                for (int t = threadIdx.x; t < cnt; t += WG_BLOCK) {
-                    const int ef = t / (nf * nm), rem = t - ef * nf * nm;
-                    const int gi = rem / nm, m = rem - gi * nm;
-                    const int gf = g0 + gi, i = tb.dir[gf];
-                    const int f1 = tb.down[3 * gf + i];
-                    const VT *R1 = K1 + (ef * nf1 + (f1 - f10)) * nm1;
-                    VT v = QC[i] * R1[m] + WQ[i] * R1[m + 1];
-                    const int cf = tb.xyz[3 * gf + i] - 1;
-                    if (cf > 0) {
-                        const int f2 = tb.down[3 * f1 + i];
-                        const VT *R2 = K2 + (ef * nf2 + (f2 - f20)) * nm2;
-                        v += (VT)cf * o2ev * (R2[m] - roe * R2[m + 1]);
-                    }
-                    const int ce = tb.xyz[3 * ef + i];
-                    if (ce > 0) {
-                        const int em = tb.down[3 * ef + i];
-                        v += (VT)ce * o2s * K1[(em * nf1 + (f1 - f10)) * nm1 + m + 1];
-                    }
-                    Kc[t] = v;
-                }
-                __syncthreads();
-                if (kl >= lc) {
-                    for (int r = 0; r < WG_NR; ++r)
-                        if (alev[r] == kl) acc[r] += f32 ? pref * (double)Kc[aoff[r]] : (double)Kc[aoff[r]];
-                }
+                wg2_levels<W, VT>(std::make_integer_sequence<int, W::Lk>{}, tb, lv, pq, acc, e0, lc);
             }
         }
         __syncthreads();  // VRR buffers are reused below
 
         // contracted (e|f), E x F, into LDS
-        for (int r = 0; r < WG_NR; ++r) {
-            const int j = threadIdx.x + WG_BLOCK * r;
-            if (j < nEF) Xb[j] = acc[r];
-        }
+        wg2_store<W>(std::make_integer_sequence<int, W::Lk>{}, acc, Xb, e0, f0, nF, lc);
+        if (lc == 0 && te < W::nAll && te >= e0) Xb[(te - e0) * nF] = acc0;
         __syncthreads();
 
         const int ncd = ncc(lc) * ncc(ld), nab = ncc(la) * ncc(lb);
@@ -458,47 +557,41 @@ __global__ void __launch_bounds__(WG_BLOCK) intc_eri_wg_k(const IntcEriArgs a, c
 
 }  // namespace
 
-// Host: the LDS layout of a class (returns -1 if it needs more than max_lds)
-int intc_wg_layout(int la, int lb, int lc, int ld, int sph, int vt_bytes, int max_lds, IntcWgCls *k)
+// Host: the LDS layout of class (LA LB | LC LD), from the kernel's own
+// compile-time sizes (returns -1 if it needs more than max_lds).  Loop phase
+// after the component tables: CH setup slots, three ket-level buffers;
+// epilogue phase (reusing that space): Z, X and the HRR buffers.
+template <int LA, int LB, int LC, int LD>
+static int wg2_layout(int sph, int vt_bytes, int max_lds, IntcWgCls *k)
 {
+    using W = Wg2<LA, LB, LC, LD>;
     IntcWgCls c{};
-    c.la = la; c.lb = lb; c.lc = lc; c.ld = ld;
-    c.L = la + lb; c.Lk = lc + ld; c.M = c.L + c.Lk; c.sph = sph;
-    if (c.L > 6 || c.Lk > 6 || c.M > 12) return -1;
-    // the kernel may flip either pair (VRR on the other shell): size every
-    // buffer for the larger of the two orientations
-    const int nE = std::max(ncc0(c.L) - ncc0(la - 1), ncc0(c.L) - ncc0(lb - 1));
-    const int nF = std::max(ncc0(c.Lk) - ncc0(lc - 1), ncc0(c.Lk) - ncc0(ld - 1));
-    c.e0 = ncc0(la - 1); c.nE = nE;
-    c.f0 = ncc0(lc - 1); c.nF = nF;
-    c.nAll = ncc0(c.L);
-    if (nE * nF > WG_NR * WG_BLOCK) return -1;
-    int o = 0;
-    for (int n = 0; n <= c.L; ++n) {
-        c.braoff[n] = o;
-        o += ncc(n) * (c.M - n + 1);
-    }
-    const int bra_total = o;
-    c.lvl_cap = c.nAll * (c.Lk + 1);
-    for (int kl = 1; kl <= c.Lk; ++kl) c.lvl_cap = std::max(c.lvl_cap, c.nAll * ncc(kl) * (c.Lk - kl + 1));
-    c.nZ = sph ? nsph(lc) * nsph(ld) : ncc(lc) * ncc(ld);
+    c.la = LA; c.lb = LB; c.lc = LC; c.ld = LD;
+    c.L = W::L; c.Lk = W::Lk; c.M = W::M; c.sph = sph;
+    // the kernel may flip either pair (VRR on the other shell): size the
+    // epilogue buffers for the larger of the two orientations
+    const int nE = std::max(ncc0(c.L) - ncc0(LA - 1), ncc0(c.L) - ncc0(LB - 1));
+    const int nF = std::max(ncc0(c.Lk) - ncc0(LC - 1), ncc0(c.Lk) - ncc0(LD - 1));
+    c.e0 = ncc0(LA - 1); c.nE = nE;
+    c.f0 = ncc0(LC - 1); c.nF = nF;
+    c.nAll = W::nAll;
+    c.lvl_cap = W::cap();
+    c.nZ = sph ? nsph(LC) * nsph(LD) : ncc(LC) * ncc(LD);
     auto hcap = [](int L, int lo, int mv) {
         int cap = 1;
         for (int j = 1; j <= mv; ++j) cap = std::max(cap, (ncc0(L - j) - ncc0(lo - 1)) * ncc(j));
         return cap;
     };
-    c.hk_cap = std::max(hcap(c.Lk, lc, ld), hcap(c.Lk, ld, lc));
-    c.hb_cap = std::max(hcap(c.L, la, lb), hcap(c.L, lb, la));
+    c.hk_cap = std::max(hcap(c.Lk, LC, LD), hcap(c.Lk, LD, LC));
+    c.hb_cap = std::max(hcap(c.L, LA, LB), hcap(c.L, LB, LA));
     c.ec = 16;
     c.cc = 16;
     auto al = [](int x) { return (x + 15) & ~15; };
     c.o_tab = 0;
     const int base = al(10 * WG_NCOMP * 4);
-    c.o_bra = base;
-    c.o_lvl = al(c.o_bra + bra_total * vt_bytes);
-    const int end1 = al(c.o_lvl + 3 * c.lvl_cap * vt_bytes);
-    // after the primitive loop: Z, then X and the ket HRR buffers; the bra
-    // HRR buffers reuse the space of X once the ket phase is done
+    c.o_bra = base;   // setup slots
+    c.o_lvl = al(c.o_bra + W::CH * W::NSLOT * 8);
+    const int end1 = al(c.o_lvl + 3 * W::cap() * vt_bytes);
     c.o_Z = base;
     c.o_X = al(c.o_Z + nE * c.nZ * 8);
     c.o_hk = al(c.o_X + nE * nF * 8);
@@ -510,29 +603,47 @@ int intc_wg_layout(int la, int lb, int lc, int ld, int sph, int vt_bytes, int ma
     return c.lds_bytes <= max_lds ? 0 : -1;
 }
 
+template <typename VT, int LA, int LB, int LC, int LD>
+static hipError_t wg2_launch(const IntcEriArgs *a, int grid, hipStream_t st)
+{
+    IntcWgCls k;
+    if (wg2_layout<LA, LB, LC, LD>(a->wg_cls[4], (int)sizeof(VT), 160 * 1024, &k)) return hipErrorInvalidValue;
+    if (a->boys_mode == 0)
+        hipLaunchKernelGGL((intc_eri_wg_k<VT, 0, LA, LB, LC, LD>), dim3(grid), dim3(WG_BLOCK), k.lds_bytes, st, *a, k);
+    else
+        hipLaunchKernelGGL((intc_eri_wg_k<VT, 1, LA, LB, LC, LD>), dim3(grid), dim3(WG_BLOCK), k.lds_bytes, st, *a, k);
+    return hipGetLastError();
+}
+
 int intc_wg_prepare()
 {
-    const void *fns[] = {(const void *)intc_eri_wg_k<double, 0>, (const void *)intc_eri_wg_k<double, 1>,
-                         (const void *)intc_eri_wg_k<float, 0>, (const void *)intc_eri_wg_k<float, 1>};
-    for (const void *f : fns)
-        if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024) != hipSuccess) return -1;
+#define INTC_WG_CLASS(A, B, C, D)                                                                            \
+    {                                                                                                        \
+        const void *fns[] = {(const void *)intc_eri_wg_k<double, 0, A, B, C, D>,                             \
+                             (const void *)intc_eri_wg_k<double, 1, A, B, C, D>,                             \
+                             (const void *)intc_eri_wg_k<float, 0, A, B, C, D>,                              \
+                             (const void *)intc_eri_wg_k<float, 1, A, B, C, D>};                             \
+        for (const void *f : fns)                                                                            \
+            if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024) != hipSuccess) \
+                return -1;                                                                                   \
+    }
+#include "gen/wg_classes.inc"
+#undef INTC_WG_CLASS
     return 0;
 }
 
+// the workgroup kernel of the bin's canonical class (a->wg_cls)
 template <typename VT>
 static hipError_t wg_launch(const IntcEriArgs *a, int grid, int block, size_t lds, hipStream_t st)
 {
     (void)block;
     (void)lds;
-    IntcWgCls k;
-    if (intc_wg_layout(a->wg_cls[0], a->wg_cls[1], a->wg_cls[2], a->wg_cls[3], a->wg_cls[4], (int)sizeof(VT),
-                       160 * 1024, &k))
-        return hipErrorInvalidValue;
-    if (a->boys_mode == 0)
-        hipLaunchKernelGGL((intc_eri_wg_k<VT, 0>), dim3(grid), dim3(WG_BLOCK), k.lds_bytes, st, *a, k);
-    else
-        hipLaunchKernelGGL((intc_eri_wg_k<VT, 1>), dim3(grid), dim3(WG_BLOCK), k.lds_bytes, st, *a, k);
-    return hipGetLastError();
+    const int la = a->wg_cls[0], lb = a->wg_cls[1], lc = a->wg_cls[2], ld = a->wg_cls[3];
+#define INTC_WG_CLASS(A, B, C, D) \
+    if (la == A && lb == B && lc == C && ld == D) return wg2_launch<VT, A, B, C, D>(a, grid, st);
+#include "gen/wg_classes.inc"
+#undef INTC_WG_CLASS
+    return hipErrorInvalidValue;
 }
 
 extern "C" hipError_t intc_eri_wg_d_launch(const IntcEriArgs *a, int grid, int block, size_t lds, hipStream_t st)

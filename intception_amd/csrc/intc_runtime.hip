@@ -327,6 +327,8 @@ int default_team_groups(const IntcClassEntry *c, int nq, double mean_prims)
 
 int grid_for(const IntcBin &b, int prec)
 {
+    // a workgroup kernel takes one quartet per workgroup at a time
+    if (b.cls[prec]->wg) return std::max(1, std::min(b.nq, 256 * 32));
     int tpw = 64 / (b.cls[prec]->G * b.team_groups[prec]);
     int teams_per_block = (b.block / 64) * tpw;
     return std::max(1, std::min((b.nq + teams_per_block - 1) / teams_per_block, 256 * 32));

@@ -1,10 +1,14 @@
-// TEST INFRASTRUCTURE ONLY (tests/test_kernel_emulation.py): a host stand-in
-// for the HIP device subset that the generated class-kernel bodies use, so a
-// body can be compiled with g++ -fsanitize=address and run on the CPU.  One
-// OS thread per lane of one team; every cross-lane read (ds_bpermute) and
-// wave barrier is a barrier of the team's lanes, which keeps them in
-// lockstep wherever the kernel exchanges data between lanes.  The product
-// never includes this file.
+// TEST INFRASTRUCTURE ONLY (tests/test_kernel_emulation.py,
+// tests/test_wg_emulation.py): a host stand-in for the HIP device subset that
+// the generated class-kernel bodies and the workgroup kernels use, so they can
+// be compiled with g++ -fsanitize=address and run on the CPU.  One OS thread
+// per lane.  Class bodies: one team; every cross-lane read (ds_bpermute) and
+// wave barrier is a barrier of the team's lanes, which keeps them in lockstep
+// wherever the kernel exchanges data between lanes.  Workgroup kernels: one
+// 256-thread block; __syncthreads is a barrier of the block and the FP64 MFMA
+// builtin a barrier of the wave's 64 lanes around an exchange of the operands
+// (the documented v_mfma_f64_16x16x4f64 lane layout).  The product never
+// includes this file.
 #pragma once
 #include <barrier>
 #include <cmath>
@@ -49,4 +53,31 @@ inline int __double2hiint(double d) { uint64_t u; memcpy(&u, &d, 8); return (int
 inline double __hiloint2double(int hi, int lo) { uint64_t u = ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo; double d; memcpy(&d, &u, 8); return d; }
 inline float __int_as_float(int i) { float f; memcpy(&f, &i, 4); return f; }
 inline int __float_as_int(float f) { int i; memcpy(&i, &f, 4); return i; }
+// block barrier and the wave barriers of the MFMA exchange (set by the driver)
+inline std::barrier<> *g_block_bar = nullptr;
+inline std::barrier<> *g_wave_bar[4] = {nullptr, nullptr, nullptr, nullptr};
+inline double g_mfma_a[256], g_mfma_b[256];
+inline void __syncthreads() { g_block_bar->arrive_and_wait(); }
+// the workgroup kernels' one vector type is four doubles (the MFMA accumulator)
+#define ext_vector_type(n) vector_size((n) * sizeof(double))
+typedef double emu_f64x4 __attribute__((vector_size(4 * sizeof(double))));
+// D[i][j] += sum_k A[i][k] B[k][j] over the wave: lane l holds A[l % 16][l / 16],
+// B[l / 16][l % 16] and D[l / 16 + 4 v][l % 16], v = 0..3
+inline emu_f64x4 emu_mfma_f64_16x16x4(double a, double b, emu_f64x4 c)
+{
+    const int t = threadIdx.x, l = t & 63, w = t >> 6, base = t - l;
+    g_mfma_a[t] = a;
+    g_mfma_b[t] = b;
+    g_wave_bar[w]->arrive_and_wait();
+    const int j = l % 16;
+    emu_f64x4 d = c;
+    for (int v = 0; v < 4; ++v) {
+        const int i = l / 16 + 4 * v;
+        for (int k = 0; k < 4; ++k) d[v] = std::fma(g_mfma_a[base + i + 16 * k], g_mfma_b[base + j + 16 * k], d[v]);
+    }
+    g_wave_bar[w]->arrive_and_wait();
+    return d;
+}
+#define __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, x, y, z) emu_mfma_f64_16x16x4((a), (b), (c))
+inline int min(int a, int b) { return a < b ? a : b; }
 using std::fma; using std::sqrt; using std::exp; using std::round; using std::fmax; using std::fabs;

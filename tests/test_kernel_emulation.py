@@ -59,12 +59,12 @@ def _build(tmp, cls, prec, bodies):
     return exe, meta
 
 
-def _inputs(cls, nq, seed, T, unit):
+def _inputs(cls, nq, seed, T, unit, sph=True):
     """Random quartets in canonical form, one bra and one ket pair each, and
     the pair builder's arrays (intc_build_pairs, computed on the host)."""
     la, lb, lc, ld = cls
     rng = np.random.default_rng(seed)
-    b = basis.make_random_quartets(seed, np.array([cls] * nq, dtype=np.int32), [2] if T > 1 else [1, 2, 3], True,
+    b = basis.make_random_quartets(seed, np.array([cls] * nq, dtype=np.int32), [2] if T > 1 else [1, 2, 3], sph,
                                    box=4.0)
     q = b.quartets
     if unit:
@@ -96,7 +96,7 @@ def _inputs(cls, nq, seed, T, unit):
             kap = 5.914967172795612878 * np.exp(-al * be / z * np.sum((A[p] - B[p]) ** 2)) * ca * cb / z
             for c, v in zip(cols, (z, 0.5 / z, P[0], P[1], P[2], kap)):
                 c.append(v)
-    ns = [basis.nsph(l) for l in cls]
+    ns = [basis.nsph(l) if sph else basis.ncc(l) for l in cls]
     blk = int(np.prod(ns))
     st = [ns[1] * ns[2] * ns[3], ns[2] * ns[3], ns[3], 1]
     qstr = sum(int(s) << (16 * k) for k, s in enumerate(st))
