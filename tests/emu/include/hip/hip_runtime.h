@@ -21,6 +21,7 @@
 #define __host__
 #define __global__
 #define __forceinline__ inline
+#define __noinline__ __attribute__((noinline))
 #define __constant__
 #define __shared__
 #define __launch_bounds__(x)
@@ -80,4 +81,6 @@ inline emu_f64x4 emu_mfma_f64_16x16x4(double a, double b, emu_f64x4 c)
 }
 #define __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, x, y, z) emu_mfma_f64_16x16x4((a), (b), (c))
 inline int min(int a, int b) { return a < b ? a : b; }
+struct uint2 { unsigned x, y; };
+inline uint2 make_uint2(unsigned x, unsigned y) { return uint2{x, y}; }
 using std::fma; using std::sqrt; using std::exp; using std::round; using std::fmax; using std::fabs;

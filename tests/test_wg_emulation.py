@@ -53,7 +53,7 @@ DEFAULT = [
 
 def _cases():
     if os.environ.get("INTC_EMU_ALL"):
-        return [(c, p, 1, 1, 0) for c in _wg_all() for p in ("d", "f")]
+        return [(c, p, s, 1, 0) for c in _wg_all() for p in ("d", "f") for s in (1, 0)]
     return DEFAULT
 
 
