@@ -108,6 +108,17 @@ def build_c99(lmax=2, jobs=None, verbose=True):
     return C99_LIB
 
 
+def class_sources(files, metas):
+    """(source, dependencies) of every generated kernel file.  The persistent
+    multi-class source includes every member's class source, so those are its
+    dependencies too: else a changed body would stay stale in it while the
+    runtime sizes its LDS from the new class table."""
+    from . import codegen
+    common = [os.path.join(CSRC, "intc_common.h")]
+    members = [os.path.join(GEN, m["name"] + ".hip") for m in codegen.multi_members(metas)]
+    return [(f, common + members if os.path.basename(f) == "intc_eri_multi.hip" else common) for f in files]
+
+
 def build(lmax=2, jobs=None, verbose=True, force=False):
     """Generate the class kernels, compile every object (all of them when
     ``force``, else only those whose sources or flags changed) and link
@@ -120,7 +131,7 @@ def build(lmax=2, jobs=None, verbose=True, force=False):
     wg = codegen.wg_classes(3, classes)
     files, metas = codegen.generate(GEN, classes, wg=wg, occupancy=OCCUPANCY, batch=BATCH)
     common = [os.path.join(CSRC, "intc_common.h")]
-    srcs = [(f, common) for f in files]
+    srcs = class_sources(files, metas)
     srcs.append((os.path.join(CSRC, "intc_runtime.hip"),
                  common + [inc, os.path.join(GEN, "class_table.inc")]))
     srcs.append((os.path.join(CSRC, "intc_mlp.hip"), []))

@@ -159,6 +159,27 @@ def slot_stride(G, ns, ipa=None):
     return best[1]
 
 
+def slot_stride_f(G, ns, ipa=None):
+    """Float-slot length (even, so the prefactor double at floats 0-1 stays
+    8-byte aligned) for the FP32 kernels' batched setup: the smallest length
+    >= ns with the fewest LDS bank conflicts for the wave's reads at a step
+    (slot group * G + st, one dword each; the bra chain's PA / WP reads, four
+    dwords from ipa per group)."""
+    ng = 64 // G
+    best = None
+    for n in range(ns + (ns & 1), ns + 18, 2):
+        worst = 0
+        for st in range(G):
+            for i in range(ns):
+                worst = max(worst, _lds_conflict([(g * G + st) * n + i for g in range(ng)]))
+            if ipa is not None:
+                ad = [(g * G + st) * n + ipa + k for g in range(ng) for k in range(4)]
+                worst = max(worst, _lds_conflict(ad))
+        if best is None or worst < best[0]:
+            best = (worst, n)
+    return best[1]
+
+
 def kernel_name(cls, spherical, prec="d"):
     return "intc_eri_%s%d%d%d%d_%s" % (prec, *cls, "s" if spherical else "c")
 
@@ -313,7 +334,23 @@ def emit_class_source(cls, spherical, prec="d", waves_per_eu=None, batch=None):
     if prec == "f":
         slot.append(("pref", "double"))
     names = [nm for nm, _ in slot]
-    NS = slot_stride(G, len(slot), names.index("PAx") if LX > 0 else None) if batched else 0
+    # FP32 recurrences: a float slot, the prefactor as one aligned double at
+    # floats 0-1 and every other entry one float (o2z and o2e as well: the
+    # recurrences read them as floats), so a step reads half the bytes and
+    # converts nothing.  NS stays the slot length in doubles (LDS sizing).
+    fslot = batched and prec == "f"
+    foff, NSF = {}, 0
+    if fslot:
+        pos = 2
+        foff["pref"] = 0
+        for nm in names:
+            if nm != "pref":
+                foff[nm] = pos
+                pos += 1
+        NSF = slot_stride_f(G, pos, foff["PAx"] if LX > 0 else None)
+        NS = NSF // 2
+    else:
+        NS = slot_stride(G, len(slot), names.index("PAx") if LX > 0 else None) if batched else 0
 
     # The body is shared by the per-class kernels below and by the persistent
     # multi-class kernel, which includes this file with INTC_BODIES_ONLY
@@ -343,6 +380,8 @@ def emit_class_source(cls, spherical, prec="d", waves_per_eu=None, batch=None):
     o("{")
     o.ind = 1
     o("constexpr int G = %d, NE = %d, NCD = %d, M = %d, NS = %d;" % (G, NE, NCD, M, NS))
+    if fslot:
+        o("constexpr int NSF = %d;  // float-slot length" % NSF)
     o("const int T = a.team_groups, TL = G * T, TPW = 64 / TL;")
     o("const int lane = threadIdx.x & 63;")
     o("const int tw = lane / TL, tl = lane - tw * TL;")
@@ -475,8 +514,12 @@ def emit_class_source(cls, spherical, prec="d", waves_per_eu=None, batch=None):
     else:
         o("// this group's primitive quartets are p = grp + k T, k < K")
         o("const int K = P > grp ? (P - grp + T - 1) / T : 0;")
-        o("double *myslot = wlds + (size_t)lane * NS;")
-        o("const double *gslot = wlds + (size_t)(lane - e) * NS;")
+        if fslot:
+            o("float *myslot = (float *)wlds + (size_t)lane * NSF;")
+            o("const float *gslot = (const float *)wlds + (size_t)(lane - e) * NSF;")
+        else:
+            o("double *myslot = wlds + (size_t)lane * NS;")
+            o("const double *gslot = wlds + (size_t)(lane - e) * NS;")
         o("for (int kb = 0; kb < K; kb += G) {")
         o.ind = 3
         o("if (kb + e < K) {")
@@ -485,8 +528,14 @@ def emit_class_source(cls, spherical, prec="d", waves_per_eu=None, batch=None):
         o("const int ib = pq / Pk, ik = pq - ib * Pk;")
         emit_setup(False)
         for i, (nm, ty) in enumerate(slot):
-            val = {"ONE": "1.0", "ZERO": "0.0"}.get(nm, "(double)" + nm)
-            o("myslot[%d] = %s;" % (i, val))
+            if fslot and nm == "pref":
+                o("*(double *)(myslot + %d) = pref;" % foff[nm])
+            elif fslot:
+                val = {"ONE": "1.0f", "ZERO": "0.0f"}.get(nm, "(float)" + nm)
+                o("myslot[%d] = %s;" % (foff[nm], val))
+            else:
+                val = {"ONE": "1.0", "ZERO": "0.0"}.get(nm, "(double)" + nm)
+                o("myslot[%d] = %s;" % (i, val))
         o.ind = 3
         o("}")
         o("__builtin_amdgcn_fence(__ATOMIC_SEQ_CST, \"wavefront\");")
@@ -494,23 +543,32 @@ def emit_class_source(cls, spherical, prec="d", waves_per_eu=None, batch=None):
         o("const int ns = (K - kb < G) ? K - kb : G;")
         o("for (int st = 0; st < ns; ++st) {")
         o.ind = 4
-        o("const double *u = gslot + st * NS;")
+        o("const %s *u = gslot + st * %s;" % (("float", "NSF") if fslot else ("double", "NS")))
         # PA / WP are read per lane by the bra chain (its step direction)
         lane_read = {"PAx", "PAy", "PAz", "ONE", "WPx", "WPy", "WPz", "ZERO"} if SLOT_PW else {"ONE", "ZERO"}
         for i, (nm, ty) in enumerate(slot):
             if nm in lane_read:
                 continue
-            if ty == "double":
+            if fslot and nm == "pref":
+                o("const double pref = *(const double *)(u + %d);" % foff[nm])
+            elif fslot and nm in ("o2z", "o2e"):
+                o("const float %sf = u[%d];" % (nm, foff[nm]))
+            elif fslot:
+                o("const float %s = u[%d];" % (nm, foff[nm]))
+            elif ty == "double":
                 o("const double %s = u[%d];" % (nm, i))
             else:
                 o("const %s %s = (%s)u[%d];" % (ty, nm, ty, i))
-        if prec == "f":
+        if prec == "f" and not fslot:
             if LX > 0:
                 o("const float o2zf = (float)o2z;")
             if LR > 0:
                 o("const float o2ef = (float)o2e;")
         names = [nm for nm, _ in slot]
-        emit_recurrences((names.index("PAx"), names.index("WPx")) if LX > 0 and SLOT_PW else None)
+        if fslot:
+            emit_recurrences((foff["PAx"], foff["WPx"]) if LX > 0 and SLOT_PW else None)
+        else:
+            emit_recurrences((names.index("PAx"), names.index("WPx")) if LX > 0 and SLOT_PW else None)
         o.ind = 3
         o("}")
         o("__builtin_amdgcn_fence(__ATOMIC_SEQ_CST, \"wavefront\");")
@@ -668,7 +726,7 @@ def emit_class_source(cls, spherical, prec="d", waves_per_eu=None, batch=None):
     o("    return hipGetLastError();")
     o("}")
     o("#endif  // INTC_BODIES_ONLY")
-    meta = dict(cls=cls, spherical=spherical, G=G, NE=NE, NCD=NCD, M=M, NS=NS, name=name, prec=prec)
+    meta = dict(cls=cls, spherical=spherical, G=G, NE=NE, NCD=NCD, M=M, NS=NS, NSF=NSF, name=name, prec=prec)
     return o.text() + "\n", meta
 
 

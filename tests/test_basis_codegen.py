@@ -102,6 +102,48 @@ def test_slot_stride_avoids_bank_conflicts():
     assert _lds_conflict([0, 64, 128, 1]) == 3
 
 
+def test_float_slot_layout():
+    # FP32 kernels: float slots of even length (the prefactor double at floats
+    # 0-1 stays 8-byte aligned), no worse in bank conflicts than any even
+    # length up to 16 longer; the generated step reads floats, converts none,
+    # and the LDS size in doubles covers 64 float slots
+    from intception_amd.codegen import _lds_conflict, slot_stride_f
+
+    def worst(G, n, ns, ipa):
+        w = 0
+        for st in range(G):
+            for i in range(ns):
+                w = max(w, _lds_conflict([(g * G + st) * n + i for g in range(64 // G)]))
+            w = max(w, _lds_conflict([(g * G + st) * n + ipa + k for g in range(64 // G) for k in range(4)]))
+        return w
+    for G in (4, 10, 20):
+        for ns in range(18, 30):
+            n = slot_stride_f(G, ns, 6)
+            assert n % 2 == 0 and ns <= n <= ns + 17
+            assert worst(G, n, ns, 6) <= min(worst(G, m, ns, 6) for m in range(ns + (ns & 1), ns + 18, 2))
+    src, meta = codegen.emit_class_source((1, 1, 2, 1), True, "f")
+    assert meta["NSF"] % 2 == 0 and meta["NS"] * 2 == meta["NSF"]
+    step = src[src.index("const float *u = gslot + st * NSF;"):src.index("acc4 = fma(pref")]
+    assert "(float)u[" not in step.replace("(float)u[8 + d", "").replace("(float)u[12 + d", "")
+    assert "const double pref = *(const double *)(u + 0);" in step
+    src, meta = codegen.emit_class_source((1, 1, 2, 1), True, "d")
+    assert meta["NSF"] == 0 and "const double *u = gslot + st * NS;" in src
+
+
+def test_multi_source_depends_on_every_member():
+    # the persistent kernel's source includes the class sources: a changed
+    # body must rebuild it
+    from intception_amd import build as bld
+    metas = [dict(name="intc_eri_d0000_s", spherical=True), dict(name="intc_eri_f1121_s", spherical=True),
+             dict(name="intc_eri_d0000_c", spherical=False)]
+    files = [os.path.join(bld.GEN, n + ".hip") for n in ("intc_eri_d0000_s", "intc_eri_f1121_s", "intc_eri_d0000_c",
+                                                          "intc_eri_multi")]
+    deps = dict((os.path.basename(f), d) for f, d in bld.class_sources(files, metas))
+    common = os.path.join(bld.CSRC, "intc_common.h")
+    assert deps["intc_eri_d0000_s.hip"] == [common]
+    assert deps["intc_eri_multi.hip"] == [common, files[0], files[1]]
+
+
 def test_flops_model_small_classes():
     # (ps|ss): a-build of 3 components at m = 0, 2 terms each
     assert flops_model.vrr_terms(1, 0, 0, 0) == 3 * 2
