@@ -320,9 +320,11 @@ __device__ __forceinline__ void wg3_epi_any(bool fb, bool fk, const IntcWgCls &k
 //   ket level kl stores (e|f)^(m) for |f| = kl, m <= Lk - kl and |e| >= elev(kl),
 //   the lowest |e| a later level or the accumulation can read:
 //   elev(kl) = max(0, min(LA, LB) - (Lk - kl)) (either orientation of the pair);
-//   layout [e - e0(kl)][f - ncc0(kl - 1)][m]; level 0 is (e|0)^(m), m <= Lk.
-//   Work items of level kl are its (e, f) pairs, item j = e_rel * nf + f_rel,
-//   each computing its m-vector; thread t takes items t + 256 r.
+//   item j = (e - e0(kl)) * nf + (f - ncc0(kl - 1)), stored plane-major,
+//   value (j, m) at m * items(kl) + j, so the 64 lanes of a wave that handle
+//   consecutive items touch consecutive addresses (no bank conflicts on the
+//   stores, few on the reads); level 0 is (e|0)^(m), m <= Lk.  Thread t
+//   computes the m-vectors of items t + 256 r.
 template <int LA, int LB, int LC, int LD>
 struct Wg2 {
     static constexpr int L = LA + LB, Lk = LC + LD, M = L + Lk;
@@ -369,7 +371,7 @@ struct Wg2 {
 };
 
 // Item descriptor of ket level KL, rep r of this thread (computed once per
-// workgroup from the component tables), element offsets within the levels:
+// workgroup from the component tables), item indices within the levels:
 //   x: R1 = (e|f-1_i) in level KL-1 (12 bits), R3 = (e-1_i|f-1_i)^(m+1) in
 //      level KL-1 (12 bits), i = dir(f) (2 bits), f_i - 1 (3 bits), e_i (3 bits)
 //   y: R2 = (e|f-2_i) in level KL-2 (12 bits), e (7 bits), f - ncc0(KL-1)
@@ -381,7 +383,7 @@ __device__ __forceinline__ void wg3_desc_level(const WgTab &tb, uint2 *d)
 {
     constexpr int NF = W::nf(KL);
     constexpr int NF1 = W::nf(KL - 1), NM1 = W::nm(KL - 1);
-    constexpr int NF2 = KL >= 2 ? W::nf(KL - 2) : 1, NM2 = KL >= 2 ? W::nm(KL - 2) : 1;
+    constexpr int NF2 = KL >= 2 ? W::nf(KL - 2) : 1;
     constexpr int E0 = W::e0(KL), E01 = W::e0(KL - 1), E02 = KL >= 2 ? W::e0(KL - 2) : 0;
     constexpr int G0 = ncc0(KL - 1), F10 = ncc0(KL - 2), F20 = ncc0(KL - 3);
     static_assert(W::cap() < 4096 && W::nAll <= 128 && NF <= 32, "descriptor fields");
@@ -396,9 +398,9 @@ __device__ __forceinline__ void wg3_desc_level(const WgTab &tb, uint2 *d)
             const int f1 = tb.down[3 * gf + i];
             const int cf = tb.xyz[3 * gf + i] - 1;
             const int ce = tb.xyz[3 * ef + i];
-            const int o1 = ((ef - E01) * NF1 + (f1 - F10)) * NM1;
-            const int o3 = ce > 0 ? ((tb.down[3 * ef + i] - E01) * NF1 + (f1 - F10)) * NM1 + 1 : 1;
-            const int o2 = (KL >= 2 && cf > 0) ? ((ef - E02) * NF2 + (tb.down[3 * f1 + i] - F20)) * NM2 : 0;
+            const int o1 = (ef - E01) * NF1 + (f1 - F10);
+            const int o3 = ce > 0 ? (tb.down[3 * ef + i] - E01) * NF1 + (f1 - F10) : 0;
+            const int o2 = (KL >= 2 && cf > 0) ? (ef - E02) * NF2 + (tb.down[3 * f1 + i] - F20) : 0;
             x = (unsigned)o1 | ((unsigned)o3 << 12) | ((unsigned)i << 24) | ((unsigned)cf << 26) | ((unsigned)ce << 29);
             y = (unsigned)o2 | ((unsigned)ef << 12) | ((unsigned)fi << 19);
         }
@@ -425,6 +427,8 @@ __device__ __forceinline__ void wg3_level(VT *const lv[3], const VT *kc, VT roe,
                                           const uint2 *d, double *acc, int e0o, int lco)
 {
     constexpr int NM = W::nm(KL);
+    // plane-major levels: (item, m) at m * items + item
+    constexpr int I0 = W::items(KL), I1 = W::items(KL - 1), I2 = KL >= 2 ? W::items(KL - 2) : 1;
     VT *Kc = lv[KL % 3];
     const VT *K1 = lv[(KL - 1) % 3];
     const VT *K2 = lv[(KL + 1) % 3];   // level KL - 2
@@ -434,24 +438,24 @@ __device__ __forceinline__ void wg3_level(VT *const lv[3], const VT *kc, VT roe,
         if (WG_BLOCK * (r + 1) <= W::items(KL) || j < W::items(KL)) {
             const unsigned x = d[W::rep_base(KL) + r].x, y = d[W::rep_base(KL) + r].y;
             const VT *R1 = K1 + (x & 0xfffu);
-            const VT *R3 = K1 + ((x >> 12) & 0xfffu);
+            const VT *R3 = K1 + ((x >> 12) & 0xfffu) + I1;   // (m+1) planes
             const int i = (int)((x >> 24) & 3u);
             const VT qc = kc[2 * i], wq = kc[2 * i + 1];
             const VT c3 = (VT)(int)(x >> 29) * o2s;
             VT v[NM];
 #pragma unroll
-            for (int m = 0; m < NM; ++m) v[m] = fma(wq, R1[m + 1], qc * R1[m]);
+            for (int m = 0; m < NM; ++m) v[m] = fma(wq, R1[(m + 1) * I1], qc * R1[m * I1]);
             if constexpr (KL >= 2) {
                 const VT *R2 = K2 + (y & 0xfffu);
                 const VT c2 = (VT)(int)((x >> 26) & 7u) * o2e;
 #pragma unroll
-                for (int m = 0; m < NM; ++m) v[m] = fma(c2, fma(-roe, R2[m + 1], R2[m]), v[m]);
+                for (int m = 0; m < NM; ++m) v[m] = fma(c2, fma(-roe, R2[(m + 1) * I2], R2[m * I2]), v[m]);
             }
 #pragma unroll
-            for (int m = 0; m < NM; ++m) v[m] = fma(c3, R3[m], v[m]);
-            VT *o = Kc + j * NM;
+            for (int m = 0; m < NM; ++m) v[m] = fma(c3, R3[m * I1], v[m]);
+            VT *o = Kc + j;
 #pragma unroll
-            for (int m = 0; m < NM; ++m) o[m] = v[m];
+            for (int m = 0; m < NM; ++m) o[m * I0] = v[m];
             if constexpr (KL >= W::acc_lo) {
                 const int ef = (int)((y >> 12) & 127u);
                 const double t = sizeof(VT) == 4 ? pref * (double)v[0] : (double)v[0];
@@ -662,9 +666,9 @@ __global__ void __launch_bounds__(WG_BLOCK) intc_eri_wg_k(const IntcEriArgs a, c
                     const int ne = cx + cy + cz;
                     wg3_chain<W::M, 0, W::L>(ca, cb, w, ne, cz, cy, o2zv, roz);
                     if (te >= W::e0(0)) {
-                        VT *o = lv[0] + (te - W::e0(0)) * (W::Lk + 1);
+                        VT *o = lv[0] + (te - W::e0(0));
 #pragma unroll
-                        for (int m = 0; m <= W::Lk; ++m) o[m] = (ne & 1) ? cb[m] : ca[m];
+                        for (int m = 0; m <= W::Lk; ++m) o[m * W::items(0)] = (ne & 1) ? cb[m] : ca[m];
                     }
                     if (lc == 0 && te >= e0)
                         acc0 += sizeof(VT) == 4 ? pref * (double)((ne & 1) ? cb[0] : ca[0])
