@@ -886,17 +886,21 @@ def run_onee(args):
 
 
 def traffic_probe(args):
-    """Counter-run helper: builds the config-3 plan, runs the plain path once,
-    then launches the FP64 and the FP32-recurrence kernel of the class
-    args.probe_class once each, and one elementwise kernel that reads 1 GiB,
-    writes 1 GiB and does 2^27 FP64 additions (the scale of the FETCH_SIZE /
-    WRITE_SIZE and FLOP counters).  Prints one JSON line."""
+    """Counter-run helper: builds the config-3 plan (--workload config2: the
+    config-2 plan), runs the plain path once, then launches the FP64 and the
+    FP32-recurrence kernel of the class args.probe_class once each, one
+    elementwise kernel that reads 1 GiB, writes 1 GiB and does 2^27 FP64
+    additions, and one 1024^3 DGEMM (the scales of the FETCH_SIZE / WRITE_SIZE,
+    VALU FLOP and MFMA counters).  Prints one JSON line."""
     import torch
-    from intception_amd import autotune, lib, runtime, screening
+    from intception_amd import autotune, basis, lib, runtime, screening
     lib.check(lib.load().intc_init())
-    mol, pf = screening.molecule_setup(200, 202603)
-    pp = screening.plan_sorted(mol, mol.screened_quartets(1e-12))
-    batch = mol.pair_batch(pp)
+    if args.workload == "config2":
+        batch = basis.config2(args.quartets2)
+    else:
+        mol, pf = screening.molecule_setup(200, 202603)
+        pp = screening.plan_sorted(mol, mol.screened_quartets(1e-12))
+        batch = mol.pair_batch(pp)
     plan = runtime.EriPlan(batch)
     out = plan.alloc_out()
     plan.run(out)
@@ -910,12 +914,14 @@ def traffic_probe(args):
     x = torch.ones(1 << 27, dtype=torch.float64, device=plan.device)
     y = torch.empty_like(x)
     torch.add(x, 1.0, out=y)
+    g = torch.randn(1024, 1024, dtype=torch.float64, device=plan.device)
+    g = g @ g
     torch.cuda.synchronize()
     nq = sum(r["nq"] for r in hit)
     plan.close()
-    return {"probe": "class kernels intc_eri_d%d%d%d%d_s_bs (FP64) then intc_eri_f%d%d%d%d_s_bs (FP32 "
-                     "recurrences) once each (%d quartets) + torch.add over 2^27 doubles (1 GiB read, 1 GiB "
-                     "written, 2^27 FP64 additions)" % (cls + cls + (nq,)), "quartets": nq}
+    return {"probe": "%s plan: the FP64 then the FP32-recurrence kernel of class (%d%d|%d%d) once each (%d "
+                     "quartets) + torch.add over 2^27 doubles (1 GiB read, 1 GiB written, 2^27 FP64 additions) + "
+                     "a 1024^3 DGEMM (2.147e9 FLOPs)" % ((args.workload,) + cls + (nq,)), "quartets": nq}
 
 
 def main():
@@ -925,7 +931,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", choices=["config3", "config1", "config2", "config4", "onee"],
                     default="config3")
-    ap.add_argument("--traffic-probe", action="store_true", help="counter-run helper (config-3 plan)")
+    ap.add_argument("--traffic-probe", action="store_true", help="counter-run helper (config-3 or config-2 plan)")
     ap.add_argument("--probe-class", default="1121", help="class of the --traffic-probe kernel")
     ap.add_argument("--quartets", type=int, default=1_000_000, help="config1 size")
     ap.add_argument("--quartets2", type=int, default=200_000, help="config2 size")
