@@ -41,7 +41,7 @@
 
 namespace {
 
-constexpr int WG_NCOMP = 120;   // components with |e| <= 7 (tables cover e + 1_i for |e| <= 6)
+constexpr int WG_NCOMP = 84;    // Cartesian components with |e| <= 6
 constexpr int WG_BLOCK = 256;
 
 typedef double f64x4 __attribute__((ext_vector_type(4)));
@@ -66,9 +66,10 @@ struct IntcWgCls {
 
 namespace {
 
-// LDS component tables: xyz, build direction (first non-zero), e - 1_i, e + 1_i
+// LDS component tables: xyz, build direction (first non-zero), e - 1_i
+constexpr int WG_TAB_INTS = 7 * WG_NCOMP;
 struct WgTab {
-    int *xyz, *dir, *down, *up;
+    int *xyz, *dir, *down;
 };
 
 __device__ __forceinline__ int cflat(int x, int y, int z)
@@ -97,8 +98,6 @@ __device__ void wg_tables(int *base)
             int e[3] = {c[0], c[1], c[2]};
             e[d] -= 1;
             base[4 * WG_NCOMP + 3 * t + d] = c[d] > 0 ? cflat(e[0], e[1], e[2]) : -1;
-            e[d] += 2;
-            base[7 * WG_NCOMP + 3 * t + d] = n < 7 ? cflat(e[0], e[1], e[2]) : -1;
         }
     }
 }
@@ -329,6 +328,7 @@ template <int LA, int LB, int LC, int LD>
 struct Wg2 {
     static constexpr int L = LA + LB, Lk = LC + LD, M = L + Lk;
     static constexpr int nAll = ncc0(L);
+    static_assert(ncc0(L) <= WG_NCOMP && ncc0(Lk) <= WG_NCOMP, "component tables cover |e|, |f| <= 6");
     static constexpr int lminb = LA < LB ? LA : LB, lmink = LC < LD ? LC : LD;
     static constexpr int elev(int kl) { return lminb - (Lk - kl) > 0 ? lminb - (Lk - kl) : 0; }
     static constexpr int e0(int kl) { return ncc0(elev(kl) - 1); }
@@ -529,7 +529,7 @@ __global__ void __launch_bounds__(WG_BLOCK) intc_eri_wg_k(const IntcEriArgs a, c
     extern __shared__ __attribute__((aligned(16))) unsigned char wg_lds[];
     int *itab = (int *)(wg_lds + k.o_tab);
     wg_tables(itab);
-    WgTab tb{itab, itab + 3 * WG_NCOMP, itab + 4 * WG_NCOMP, itab + 7 * WG_NCOMP};
+    WgTab tb{itab, itab + 3 * WG_NCOMP, itab + 4 * WG_NCOMP};
     double *slots = (double *)(wg_lds + k.o_bra);
     VT *const lv[3] = {(VT *)(wg_lds + k.o_lvl), (VT *)(wg_lds + k.o_lvl) + W::cap(),
                        (VT *)(wg_lds + k.o_lvl) + 2 * W::cap()};
@@ -740,7 +740,7 @@ static int wg2_layout(int sph, int vt_bytes, int max_lds, IntcWgCls *k)
         wg3_epi_sizes<LA, LB, LC, LD, false>(&nX, &nT, &nZ, &nH);
     auto al = [](int x) { return (x + 15) & ~15; };
     c.o_tab = 0;
-    c.o_S = al(10 * WG_NCOMP * 4);
+    c.o_S = al(WG_TAB_INTS * 4);
     const int base = al(c.o_S + intc_sph_off(LMAX + 1) * 8);
     c.o_bra = base;
     c.o_lvl = al(c.o_bra + W::CH * W::NSLOT * 8);
