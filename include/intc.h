@@ -156,6 +156,17 @@ int intc_plan_graph_launch(intc_plan *plan, void *stream);
  * route code 1 as FP64 and launches the FP64 kernels only (no effect on
  * graphs already captured). */
 int intc_plan_set_route_fp32(intc_plan *plan, int use_fp32);
+/* FP32 error check of routed runs (no counterpart in the reference, whose
+ * generated C is FP64 throughout): with tol > 0 every FP32-recurrence register
+ * kernel estimates its quartet's error as 8 * 2^(la+lb+lc+ld) * 2^-24 times
+ * the largest contracted (e 0|f 0) intermediate and queues a quartet whose
+ * estimate exceeds tol; the bin's FP64 kernel then recomputes the queued
+ * quartets on the same stream, overwriting their blocks.  tol = 0 turns it off
+ * (the default).  Graphs keep the setting they were captured with.  -1 for a
+ * negative or non-finite tol. */
+int intc_plan_set_fp32_check(intc_plan *plan, double tol);
+/* Per bin: quartets the last routed run recomputed in FP64 after the check. */
+int intc_plan_redo_counts(intc_plan *plan, int *counts);
 /* Persistent multi-class launch (no counterpart in the reference): with
  * enable = 1, execute, execute_routed and graphs captured afterwards run every
  * bin whose class the multi-class kernel holds (all spherical register

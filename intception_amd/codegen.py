@@ -126,6 +126,9 @@ BATCH_LK = (2, 3)
 # batched kernels: each lane reads its bra-chain step's PA / WP entry from the
 # setup slot (instead of selecting among the x, y, z entries)
 SLOT_PW = True
+# the top ket level's cross-lane term is added once per contracted quartet
+# (_ket_lines defer_top) instead of once per primitive quartet
+DEFER_TOP = True
 
 
 def _lds_conflict(addrs):
@@ -213,8 +216,15 @@ def _chain_lines(LX, M, R="double", slot_pw=None):
     return L
 
 
-def _ket_lines(LX, LR, R="double"):
+def _ket_lines(LX, LR, R="double", defer_top=False):
     """Ket OS VRR in registers: V{f}_{m} for |f| <= LR, m <= LR-|f| (type R).
+
+    defer_top: the top level |f| = LR leaves out its cross-lane term
+    e_i/(2(zeta+eta)) (e-1_i 0|f-1_i 0)^(1).  That level only feeds the
+    contraction sums, so the term is linear in them: the caller accumulates
+    o2s (e 0|f' 0)^(1) per lane (f' of level LR-1) over the primitive quartets
+    and adds e_i times the sum of lane e-1_i once per contracted quartet
+    (top_coupling).
 
     Returns (lines, name(f, m))."""
     L = []
@@ -236,10 +246,28 @@ def _ket_lines(LX, LR, R="double"):
                 if k1 > 0:
                     f2 = dec(f1, i)
                     expr += " + %d.0%s * o2e * (%s - roe * %s)" % (k1, lit, V(f2, m), V(f2, m + 1))
-                if LX > 0:
+                if LX > 0 and not (defer_top and n == LR):
                     expr += " + cpl%s * %s(%s, s4%s)" % (ax, shfl, V(f1, m + 1), ax)
                 L.append("const %s %s = %s;" % (R, V(t, m), expr))
     return L, V
+
+
+def fp32_err_c(cls):
+    """Constant C of the FP32 kernels' error estimate C 2^-24 max|(e 0|f 0)|:
+    8 * 2^(la+lb+lc+ld).  On quartets of the config-3 molecule's shells the
+    largest measured ratio of the FP32 error to 2^-24 max|(e 0|f 0)| was 8
+    to 130 by class (oracle reference), at least 2x below this constant."""
+    return 8.0 * 2.0 ** sum(cls)
+
+
+def top_coupling(LR):
+    """The deferred top-level terms: (target t, direction i, source f1 = t - 1_i)
+    for every |t| = LR, i = first non-zero of t."""
+    out = []
+    for t in cart_components(LR):
+        i = first_nonzero(t)
+        out.append((t, i, dec(t, i)))
+    return out
 
 
 def _hrr_lines(src, lo, nmove, shift_vec, prefix, kind):
@@ -433,6 +461,12 @@ def emit_class_source(cls, spherical, prec="d", waves_per_eu=None, batch=None):
     acc_f = [f for f in ket_comps if sum(f) >= LC]
     for f in acc_f:
         o("double acc%d = 0.0;" % cidx(f))
+    # deferred top-level coupling: per lane, sum over primitive quartets of
+    # o2s (e 0|f1 0)^(1) for every f1 of level LR - 1 (FP64, prefactor included)
+    defer = DEFER_TOP and LX > 0 and LR > 0
+    defer_src = cart_components(LR - 1) if defer else []
+    for f1 in defer_src:
+        o("double bq%d = 0.0;" % cidx(f1))
     def emit_setup(load_idx):
         """pair loads + Boys + coefficients for primitive quartet (ib, ik)"""
         o("const double zeta = a.pp_zeta[ob + ib], o2z = a.pp_o2z[ob + ib];")
@@ -489,19 +523,26 @@ def emit_class_source(cls, spherical, prec="d", waves_per_eu=None, batch=None):
             o("const %s cplx = ex_d * o2s, cply = ey_d * o2s, cplz = ez_d * o2s;" % R)
         if prec == "f":
             body = _chain_lines(LX, M, R, slot_pw)
-            lines, V = _ket_lines(LX, LR, R)
+            lines, V = _ket_lines(LX, LR, R, defer)
             body += lines
             body = [ln.replace("o2z", "o2zf").replace("o2e", "o2ef") for ln in body]
             for ln in body:
                 o(ln)
             for f in acc_f:
                 o("acc%d = fma(pref, (double)%s, acc%d);" % (cidx(f), V(f, 0), cidx(f)))
+            if defer:
+                # o2s and the m = 1 values carry the FP32 scale 1/s and s
+                o("const double po2s = pref * (double)o2s;")
+                for f1 in defer_src:
+                    o("bq%d = fma(po2s, (double)%s, bq%d);" % (cidx(f1), V(f1, 1), cidx(f1)))
         else:
             for ln in _chain_lines(LX, M, "double", slot_pw):
                 o(ln)
-            lines, V = _ket_lines(LX, LR)
+            lines, V = _ket_lines(LX, LR, "double", defer)
             for ln in lines:
                 o(ln)
+            for f1 in defer_src:
+                o("bq%d = fma(o2s, %s, bq%d);" % (cidx(f1), V(f1, 1), cidx(f1)))
             for f in acc_f:
                 o("acc%d += %s;" % (cidx(f), V(f, 0)))
 
@@ -582,8 +623,35 @@ def emit_class_source(cls, spherical, prec="d", waves_per_eu=None, batch=None):
     o("const bool take = ((grp & (2 * s - 1)) == 0) && (grp + s < T);")
     for f in acc_f:
         o("{ const double r = intc_shfl(acc%d, src4); if (take) acc%d += r; }" % (cidx(f), cidx(f)))
+    for f1 in defer_src:
+        o("{ const double r = intc_shfl(bq%d, src4); if (take) bq%d += r; }" % (cidx(f1), cidx(f1)))
     o.ind = 2
     o("}")
+    if defer:
+        # the deferred top-level term, every lane (cross-lane reads of group 0's sums)
+        o("// top ket level: + e_i (sum of o2s (e-1_i 0|t-1_i 0)^(1)) from lane e - 1_i")
+        for t, i, f1 in top_coupling(LR):
+            ax = "xyz"[i]
+            o("acc%d = fma((double)e%s_d, intc_shfl(bq%d, s4%s), acc%d);" % (cidx(t), ax, cidx(f1), ax, cidx(t)))
+    if prec == "f":
+        # a-posteriori error estimate of the FP32 recurrences: C 2^-24 times
+        # the largest contracted (e 0|f 0) of the quartet (FP32_ERR_C: the
+        # rounding of the recurrence values, amplified by the contraction, the
+        # HRR and the transforms, stays below that); over the tolerance the
+        # quartet goes to the FP64 kernel's redo queue
+        o("if (a.fp32_tol > 0.0) {")
+        o.ind += 1
+        o("float mx = 0.0f;")
+        o("if (grp == 0 && e >= %d) {" % eoff)
+        # a NaN or infinite intermediate reads as infinite (always redone)
+        for f in acc_f:
+            o("    { const float v = (float)fabs(acc%d); mx = fmaxf(mx, v < 3.0e38f ? v : INFINITY); }" % cidx(f))
+        o("}")
+        o("for (int s = 1; s < G; s <<= 1) mx = fmaxf(mx, intc_shflf(mx, (e + s < G ? lane + s : lane) * 4));")
+        o("if (grp == 0 && e == 0 && (double)mx * %.17g > a.fp32_tol) a.redo_q[atomicAdd(a.redo_n, 1)] = q;"
+          % (fp32_err_c(cls) * 2.0 ** -24))
+        o.ind -= 1
+        o("}")
     o("#ifdef INTC_BODIES_ONLY")
     o("const long long qoff = a.q_off[q];")
     o("const unsigned long long qs = a.q_str[q];")

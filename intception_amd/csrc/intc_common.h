@@ -44,6 +44,12 @@ struct IntcEriArgs {
     double *out;
     int wg_cls[6];                    // workgroup kernels: la, lb, lc, ld, spherical (canonical class)
     int sph_builtin;                  // 1: sph is the built-in table (register kernels use literals)
+    // FP32-recurrence register kernels, routed runs: a quartet whose error
+    // estimate exceeds fp32_tol is appended to redo_q (redo_n counts them) for
+    // the FP64 kernel that runs after it; fp32_tol <= 0 turns the estimate off
+    int *redo_q;
+    int *redo_n;
+    double fp32_tol;
 };
 
 // One-electron integrals (overlap, nuclear attraction) of contracted shell

@@ -248,6 +248,11 @@ struct intc_plan {
     double prim_cut = 0.0;
     int *pair_np_scr = nullptr;
     bool route_fp32 = true;     // routed runs launch the FP32-recurrence queues
+    // FP32 error check: the FP32 register kernels append quartets whose error
+    // estimate exceeds fp32_tol to redo_q (bin-local positions, at bin_q0;
+    // redo_n per bin) and each bin's FP64 kernel then recomputes them
+    double fp32_tol = 0.0;
+    int *redo_q = nullptr, *redo_n = nullptr;
     // persistent multi-class launch (intc_plan_set_persistent): the bins whose
     // class the multi kernel holds run as tiles of one launch; one buffer set
     // for eager execution and one bound by the captured graph
@@ -287,7 +292,7 @@ void plan_free(intc_plan *p)
                     p->pair_sb, p->pair_off, p->pair_np, p->pair_nk, p->pp_pair, p->shell_off, p->coef_off, p->cen_idx, p->idx64,
                     p->idx32, p->idxs, p->counts, p->tile_q0, p->tile_n, p->tile_bin, p->bin_tile0,
                     p->tile_cnt, p->tile_base, p->bin_q0, p->pz, p->po2z, p->ppx, p->ppy, p->ppz, p->pkap,
-                    p->pairA, p->pairAB, p->pair_np_scr};
+                    p->pairA, p->pairAB, p->pair_np_scr, p->redo_q, p->redo_n};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
     for (intc_plan::MultiBufs *m : {&p->m_eager, &p->m_graph}) {
@@ -404,7 +409,8 @@ __global__ void __launch_bounds__(256) intc_route_count(const signed char *__res
 
 __global__ void __launch_bounds__(256) intc_route_scan(const int *__restrict__ bin_tile0,
                                                       const int *__restrict__ tile_cnt,
-                                                      int *__restrict__ tile_base, int *__restrict__ counts)
+                                                      int *__restrict__ tile_base, int *__restrict__ counts,
+                                                      int *__restrict__ redo_n)
 {
     const int b = blockIdx.x, t0 = bin_tile0[b], nt = bin_tile0[b + 1] - t0;
     int run[3] = {0, 0, 0};
@@ -417,8 +423,10 @@ __global__ void __launch_bounds__(256) intc_route_scan(const int *__restrict__ b
             for (int c = 0; c < 3; ++c) tile_base[3 * (t0 + i) + c] = run[c] + ex[c];
         for (int c = 0; c < 3; ++c) run[c] += tot[c];
     }
-    if (threadIdx.x == 0)
+    if (threadIdx.x == 0) {
         for (int c = 0; c < 3; ++c) counts[3 * b + c] = run[c];
+        if (redo_n) redo_n[b] = 0;   // the FP32 kernels of this run append from 0
+    }
 }
 
 __global__ void __launch_bounds__(256) intc_route_scatter(const signed char *__restrict__ route,
@@ -1164,7 +1172,7 @@ static int route_queues(intc_plan *p, const signed char *route, hipStream_t st)
                        p->tile_n, use32, p->tile_cnt);
     HIPCHK(hipGetLastError());
     hipLaunchKernelGGL(intc_route_scan, dim3(nb), dim3(256), 0, st, p->bin_tile0, p->tile_cnt, p->tile_base,
-                       p->counts);
+                       p->counts, p->redo_n);
     HIPCHK(hipGetLastError());
     hipLaunchKernelGGL(intc_route_scatter, dim3(p->n_tile), dim3(256), 0, st, route, p->q_orig, ident, p->tile_q0,
                        p->tile_n, p->tile_bin, p->bin_q0, p->tile_base, use32, p->idx64, p->idx32, p->idxs);
@@ -1196,6 +1204,11 @@ static IntcEriArgs bin_args(const intc_plan *p, const IntcBin &b, int prec, cons
     a.q_idx = routed ? (prec == 0 ? p->idx64 : p->idx32) + b.q0 : nullptr;
     a.nq_dev = routed ? p->counts + 3 * bi + prec : nullptr;
     a.nq = b.nq;
+    if (routed && prec == 1 && p->fp32_tol > 0.0) {
+        a.fp32_tol = p->fp32_tol;
+        a.redo_q = p->redo_q + b.q0;
+        a.redo_n = p->redo_n + bi;
+    }
     a.team_groups = b.team_groups[prec];
     const IntcClassEntry *c = b.cls[prec];
     a.wg_cls[0] = c->la;
@@ -1328,6 +1341,24 @@ static int prepare_multi(intc_plan *p, const double *sph, double *out, bool rout
     return 0;
 }
 
+// the FP32 error check applies to a bin's FP32 queue when its FP32 kernel is
+// a register kernel of its own (FP32-routed quartets of other bins run the
+// FP64 kernel, the workgroup kernels carry no estimate)
+static bool redo_applies(const intc_plan *p, const IntcBin &b)
+{
+    return p->fp32_tol > 0.0 && b.cls[1] && b.cls[1]->prec == 1 && !b.cls[1]->wg;
+}
+
+// FP64 recomputation of the quartets that a bin's FP32 launch queued (device-
+// side count; the class kernels stride over any queue length with this grid).
+// Ordered after that launch on the same stream.
+static int launch_redo(const intc_plan *p, const IntcBin &b, int bi, const IntcEriArgs &a, hipStream_t st)
+{
+    IntcBin br = b;
+    br.grid[0] = std::min(b.grid[0], 256 * 2);
+    return launch_bin(p, br, 0, a, p->redo_q + b.q0, p->redo_n + bi, st);
+}
+
 static int run_all(intc_plan *p, const double *sph, double *out, const signed char *route, int n_streams,
                    hipStream_t st, const intc_plan::MultiBufs *mb = nullptr)
 {
@@ -1363,6 +1394,12 @@ static int run_all(intc_plan *p, const double *sph, double *out, const signed ch
         if (mb->n_tiles > 0) {
             hipError_t e = intc_multi_launch(mb->args, mb->tiles, mb->n_tiles, mb->next, mb->reg, p->multi_grid, st);
             if (e != hipSuccess) return fail(-3, std::string("persistent kernel launch: ") + hipGetErrorString(e));
+            if (route && p->route_fp32)
+                for (size_t bi = 0; bi < p->bins.size(); ++bi)
+                    if (multi_covers(p, p->bins[bi], 1) && redo_applies(p, p->bins[bi])) {
+                        rc = launch_redo(p, p->bins[bi], (int)bi, a, st);
+                        if (rc) return rc;
+                    }
         }
         k = n_streams > 1 ? 1 : 0;
     }
@@ -1379,7 +1416,15 @@ static int run_all(intc_plan *p, const double *sph, double *out, const signed ch
                 // queue lengths are only known on the device: persistent grid
                 IntcBin br = b;
                 br.grid[prec] = std::min(b.grid[prec], 256 * 16);
-                rc = launch_bin(p, br, prec, a, qi, nd, s2);
+                const bool redo = prec == 1 && redo_applies(p, b);
+                IntcEriArgs a2 = a;
+                if (redo) {
+                    a2.fp32_tol = p->fp32_tol;
+                    a2.redo_q = p->redo_q + b.q0;
+                    a2.redo_n = p->redo_n + bi;
+                }
+                rc = launch_bin(p, br, prec, a2, qi, nd, s2);
+                if (!rc && redo) rc = launch_redo(p, b, bi, a, s2);
             } else {
                 rc = launch_bin(p, b, prec, a, qi, nd, s2);
             }
@@ -1536,6 +1581,36 @@ INTC_API int intc_plan_graph_launch(intc_plan *p, void *stream)
 INTC_API int intc_plan_set_route_fp32(intc_plan *p, int use_fp32)
 {
     p->route_fp32 = use_fp32 != 0;
+    return 0;
+}
+
+// FP32 error check of routed runs (tol > 0; 0 turns it off).  Set it before
+// capturing a graph: graphs keep the setting they were captured with.
+INTC_API int intc_plan_set_fp32_check(intc_plan *p, double tol)
+{
+    if (!(tol >= 0.0) || !std::isfinite(tol)) return fail(-1, "tol must be finite and >= 0 (0: off)");
+    if (tol > 0.0 && !p->redo_q) {
+        const size_t nb = std::max<size_t>(1, p->bins.size());
+        if (hipMalloc(&p->redo_q, sizeof(int) * std::max<long long>(1, p->n_quartet)) != hipSuccess ||
+            hipMalloc(&p->redo_n, sizeof(int) * nb) != hipSuccess)
+            return fail(-3, "hipMalloc FP32 redo queues");
+        HIPCHK(hipMemset(p->redo_n, 0, sizeof(int) * nb));
+    }
+    p->fp32_tol = tol;
+    return 0;
+}
+
+// per bin: quartets the last routed run recomputed in FP64 after the FP32
+// error check (host copy, synchronising; zeros when the check is off)
+INTC_API int intc_plan_redo_counts(intc_plan *p, int *counts_out)
+{
+    const int nb = (int)p->bins.size();
+    if (nb == 0) return 0;
+    if (!p->redo_n) {
+        std::fill(counts_out, counts_out + nb, 0);
+        return 0;
+    }
+    HIPCHK(hipMemcpy(counts_out, p->redo_n, sizeof(int) * nb, hipMemcpyDeviceToHost));
     return 0;
 }
 

@@ -47,6 +47,8 @@ SIGNATURES = [
                                        c_int, c_void_p]),
     ("intc_plan_graph_launch", c_int, [c_void_p, c_void_p]),
     ("intc_plan_set_route_fp32", c_int, [c_void_p, c_int]),
+    ("intc_plan_set_fp32_check", c_int, [c_void_p, c_double]),
+    ("intc_plan_redo_counts", c_int, [c_void_p, _ip]),
     ("intc_plan_set_persistent", c_int, [c_void_p, c_int, c_int, c_int]),
     ("intc_plan_persistent_info", c_int, [c_void_p, _ip, _ip]),
     ("intc_plan_set_persistent_bins", c_int, [c_void_p, c_int, _ip]),

@@ -138,6 +138,20 @@ class EriPlan:
     def set_route_fp32(self, use_fp32):
         _lib.check(self._L.intc_plan_set_route_fp32(self.handle, int(bool(use_fp32))))
 
+    def set_fp32_check(self, tol):
+        """Routed runs: the FP32-recurrence register kernels estimate each
+        quartet's error (8 * 2^(la+lb+lc+ld) * 2^-24 times its largest
+        contracted intermediate) and queue the quartets above tol for the
+        FP64 kernel, which recomputes them after the FP32 launch.  0: off."""
+        _lib.check(self._L.intc_plan_set_fp32_check(self.handle, float(tol)))
+
+    def redo_counts(self):
+        """Per bin: quartets the last routed run recomputed in FP64."""
+        n = self._L.intc_plan_num_bins(self.handle)
+        buf = (ctypes.c_int * max(n, 1))()
+        _lib.check(self._L.intc_plan_redo_counts(self.handle, buf))
+        return np.array(buf[:n])
+
     def set_persistent(self, enable=True, tile_iters=64, grid=0):
         """One persistent launch for every bin whose class the multi-class
         kernel holds (spherical, built-in transforms, Chebyshev Boys): waves

@@ -97,7 +97,13 @@ class LearnedRouter:
         """Thresholds (predicted magnitude, Schwarz product) for the FP32 route:
         on held-out quartets that are not skipped, the pair of thresholds
         (0.5-decade grid) that sends the most quartets to FP32 while every one
-        of them keeps its FP32-recurrence error within the FP32 budget."""
+        of them keeps its FP32-recurrence error within the FP32 budget, and
+        the budget still holds one grid step further in both thresholds.
+        The FP32 error has a heavy tail (the HRR and the contraction amplify
+        the rounding of the recurrence values by up to ~1e4 of the block
+        maximum), so a threshold pair picked at the held-out set's edge can
+        admit a far larger error on another molecule; the step of margin
+        keeps the chosen pair half a decade inside the region that held."""
         m = self.net.margin_32
         keep = pred + self.net.margin_skip >= self.log_skip
         keep &= logqq >= self.log_skip - 1e-4        # the route kernel also skips on the Schwarz bound
@@ -106,10 +112,12 @@ class LearnedRouter:
             sel_p = keep & (pred + m < tp)
             if not sel_p.any():
                 continue
+            wide_p = keep & (pred + m < tp + 0.5)
             for tq in np.arange(-16.0, 4.01, 0.5):
                 sel = sel_p & (logqq < tq)
                 n = int(sel.sum())
-                if n > best[0] and e32[sel].max() <= self.fp32_budget:
+                wide = wide_p & (logqq < tq + 0.5)
+                if n > best[0] and e32[wide].max() <= self.fp32_budget:
                     best = (n, tp, tq)
         _, self.log_32, self.log_qq32 = best
         return best

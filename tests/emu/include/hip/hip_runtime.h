@@ -49,6 +49,8 @@ inline int emu_bpermute(int addr, int v)
 #define __builtin_amdgcn_ds_bpermute(a, v) emu_bpermute((a), (v))
 #define __builtin_amdgcn_wave_barrier() g_team.bar->arrive_and_wait()
 #define __builtin_amdgcn_fence(a, b) ((void)0)
+// device atomics: the lanes of the team are host threads
+inline int atomicAdd(int *p, int v) { return __atomic_fetch_add(p, v, __ATOMIC_SEQ_CST); }
 inline int __double2loint(double d) { uint64_t u; memcpy(&u, &d, 8); return (int)(uint32_t)u; }
 inline int __double2hiint(double d) { uint64_t u; memcpy(&u, &d, 8); return (int)(uint32_t)(u >> 32); }
 inline double __hiloint2double(int hi, int lo) { uint64_t u = ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo; double d; memcpy(&d, &u, 8); return d; }

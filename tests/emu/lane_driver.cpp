@@ -1,6 +1,8 @@
 // TEST INFRASTRUCTURE ONLY: runs one generated class-kernel body on the CPU
 // (tests/emu/include/hip/hip_runtime.h) for tests/test_kernel_emulation.py.
-// argv: input file, output file, team size T, teams to run, routed (0/1).
+// argv: input file, output file, team size T, teams to run, routed (0/1),
+// optional FP32 error-check tolerance (the redo queue is then written after
+// the output: its count, then the queued quartet indices, as ints).
 // Input: nq, npair, npp, nout, then the IntcEriArgs arrays (quartets, pairs,
 // primitive pairs, queue); the wave's teams run one after the other on one
 // exactly sized LDS region and exactly sized device arrays, so AddressSanitizer
@@ -59,6 +61,14 @@ int main(int argc, char **argv)
     for (int i = 0; i < nout; ++i) out[i] = 12345.0;
     a.out = out;
     a.sph_builtin = 1;
+    int *redo_q = new int[nq > 0 ? nq : 1];
+    int *redo_n = new int[1];
+    redo_n[0] = 0;
+    if (argc > 6) {
+        a.fp32_tol = atof(argv[6]);
+        a.redo_q = redo_q;
+        a.redo_n = redo_n;
+    }
     const int G = CAT(KNAME, _G);
     const int TL = G * T, TPW = 64 / TL;
     const int reg = CAT(KNAME, _reg)(T);
@@ -81,6 +91,10 @@ int main(int argc, char **argv)
     }
     f = fopen(argv[2], "wb");
     fwrite(out, 8, nout, f);
+    if (argc > 6) {
+        fwrite(redo_n, 4, 1, f);
+        fwrite(redo_q, 4, redo_n[0], f);
+    }
     fclose(f);
     printf("emulated %d quartets, G=%d, T=%d, teams %d, routed %d, reg=%d doubles\n", nq, G, T, std::min(nteams, TPW), routed, reg);
     return 0;

@@ -75,9 +75,13 @@ def test_full_three_center_learned_routing(mol4):
                  route)
     codes = torch.bincount(route.to(torch.int64), minlength=3).cpu().numpy() / batch.n_quartet
     plan.set_route_fp32(True)
+    # FP32 error check at the router's FP32 budget: FP32-routed quartets whose
+    # estimate exceeds it are recomputed by the FP64 kernels
+    plan.set_fp32_check(router.fp32_budget)
     got = plan.run_routed(route)
     err = max_abs_diff(got, out64)
-    print("routed fp64/fp32/skip = %s, max abs err vs FP64 %.3e" % (np.round(codes, 4).tolist(), err), flush=True)
+    print("routed fp64/fp32/skip = %s, %d FP32 quartets redone in FP64, max abs err vs FP64 %.3e"
+          % (np.round(codes, 4).tolist(), plan.redo_counts().sum(), err), flush=True)
     assert err <= 1e-10
     plan.set_prim_screen(plan.prim_screen_cut(1e-11))
     try:

@@ -100,3 +100,51 @@ def test_random_route_and_graph(gpu, plan_sorted):
         g2 = out2.cpu().numpy()
         eager = plan.run_routed(route).cpu().numpy()
         assert np.array_equal(g2, eager), "graph replay %d differs from eager" % rep
+
+
+def test_fp32_error_check_redo(gpu):
+    """FP32 error check with every quartet routed to FP32 (config-1 classes):
+    a tolerance below every estimate sends every quartet of the FP32 register
+    kernels to the FP64 redo, so the output equals the plain FP64 output bit
+    for bit; tolerance 0 (off) gives the plain FP32 result; a tolerance of
+    1e-11 redoes the quartets whose estimate exceeds it and keeps every
+    element within 1e-11 of FP64, alone and in a captured graph."""
+    import torch
+    b = basis.config1(n_quartet=20000, seed=71)
+    plan = runtime.EriPlan(b)
+    ref = plan.run().cpu().numpy()
+    route = torch.ones(b.n_quartet, dtype=torch.int8, device=plan.device)
+    plain32 = plan.run_routed(route).cpu().numpy()
+    plan.set_fp32_check(1e-300)
+    got = plan.run_routed(route).cpu().numpy()
+    redo = plan.redo_counts()
+    counts = plan.route_counts()
+    print("tolerance 1e-300: %d of %d quartets redone in FP64" % (redo.sum(), b.n_quartet), flush=True)
+    # every quartet whose largest intermediate is a normal FP32 number is
+    # redone; the others (intermediates below 1.2e-38, where the FP32 maximum
+    # reads zero) keep their FP32 result, whose error is as small
+    diff = got != ref
+    print("  elements not equal to FP64: %d, largest |FP64| among them %.1e, largest difference %.1e, "
+          "non-finite FP32-route elements without the check %d"
+          % (diff.sum(), np.abs(ref[diff]).max() if diff.any() else 0.0,
+             np.abs(got[diff] - ref[diff]).max() if diff.any() else 0.0, (~np.isfinite(plain32)).sum()), flush=True)
+    assert not diff.any() or (np.abs(ref[diff]).max() < 1e-30 and np.abs(got[diff] - ref[diff]).max() < 1e-30)
+    assert np.all(redo <= counts[:, 1]) and redo.sum() > 0.9 * b.n_quartet
+    plan.set_fp32_check(0.0)
+    got = plan.run_routed(route).cpu().numpy()
+    assert np.array_equal(got, plain32) and plan.redo_counts().sum() == 0
+    plan.set_fp32_check(1e-11)
+    got = plan.run_routed(route).cpu().numpy()
+    n_redo = int(plan.redo_counts().sum())
+    err = np.abs(got - ref).max()
+    print("tolerance 1e-11: %d quartets redone, max abs err %.3e (plain FP32: %.3e)"
+          % (n_redo, err, np.abs(plain32 - ref).max()), flush=True)
+    assert err <= 1e-11
+    out = plan.alloc_out()
+    plan.capture(out, route=route)
+    for _ in range(3):
+        out.zero_()
+        plan.replay()
+        torch.cuda.synchronize()
+        assert np.array_equal(out.cpu().numpy(), got)
+    plan.close()

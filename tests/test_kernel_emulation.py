@@ -128,6 +128,40 @@ def _run(tmp, exes, case, nq=24, seed=5):
 
 
 @pytest.mark.skipif(GXX is None, reason="needs g++")
+def test_fp32_error_check_queue_on_the_host(tmp_path):
+    """FP32 bodies with the error check on: a tolerance below every estimate
+    queues every quartet exactly once, one above every estimate queues none,
+    and the output is the same either way (the FP64 kernel's redo is the
+    runtime's, not the body's)."""
+    tmp = str(tmp_path)
+    oracle.set_boys_mode(oracle.BOYS_EXACT)
+    for cls, T in (((1, 1, 2, 1), 1), ((1, 0, 1, 0), 2), ((2, 1, 2, 1), 1)):
+        exe, _ = _build(tmp, cls, "f", 0)
+        b, parts = _inputs(cls, 20, 9, T, 0)
+        nout = int(parts[0][3])
+        inp = os.path.join(tmp, "in_chk")
+        with open(inp, "wb") as fh:
+            for arr in parts:
+                arr.tofile(fh)
+        outs = []
+        for tol in ("1e-300", "1e300"):
+            outp = os.path.join(tmp, "out_chk")
+            r = subprocess.run([exe, inp, outp, str(T), "64", "1", tol], capture_output=True, text=True,
+                               env=dict(os.environ, ASAN_OPTIONS="detect_leaks=0:abort_on_error=1"))
+            assert r.returncode == 0, r.stderr[-3000:]
+            raw = np.fromfile(outp, dtype=np.uint8)
+            out = raw[:8 * nout].view(np.float64)
+            n = int(raw[8 * nout:8 * nout + 4].view(np.int32)[0])
+            q = raw[8 * nout + 4:].view(np.int32)
+            outs.append(out)
+            if tol == "1e-300":
+                assert n == 20 and sorted(q.tolist()) == list(range(20)), (cls, n, q)
+            else:
+                assert n == 0 and len(q) == 0, (cls, n)
+        assert np.array_equal(outs[0], outs[1])
+
+
+@pytest.mark.skipif(GXX is None, reason="needs g++")
 def test_class_bodies_on_the_host(tmp_path):
     tmp = str(tmp_path)
     keys = sorted({(c[0], c[1], c[4]) for c in CASES})
