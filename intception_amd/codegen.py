@@ -247,7 +247,14 @@ def _ket_lines(LX, LR, R="double", defer_top=False):
                     f2 = dec(f1, i)
                     expr += " + %d.0%s * o2e * (%s - roe * %s)" % (k1, lit, V(f2, m), V(f2, m + 1))
                 if LX > 0 and not (defer_top and n == LR):
-                    expr += " + cpl%s * %s(%s, s4%s)" % (ax, shfl, V(f1, m + 1), ax)
+                    if n == 1 and (LX == 1 or i == 0):
+                        # the first level's neighbour value (e - 1_i 0|0 0)^(m+1) is
+                        # the lane's own chain value before the last (the chain
+                        # builds e from e - 1_i0, i0 = first non-zero of e; i = x
+                        # is i0 wherever e_x > 0, and with |e| <= 1 every i is)
+                        expr += " + cpl%s * c%d_%d" % (ax, LX - 1, m + 1)
+                    else:
+                        expr += " + cpl%s * %s(%s, s4%s)" % (ax, shfl, V(f1, m + 1), ax)
                 L.append("const %s %s = %s;" % (R, V(t, m), expr))
     return L, V
 
@@ -328,9 +335,14 @@ def emit_class_source(cls, spherical, prec="d", waves_per_eu=None, batch=None):
             steps.append((i, t[i] - 1))
             t = dec(t, i)
         steps = steps[::-1]
+        # padding steps (direction 3: PA 1, WP 0) come first, so that the
+        # chain's value before the last is e's predecessor e - 1_i0 in every
+        # lane (i0 = first non-zero of e); the ket's first level reads it
+        # there instead of from another lane
+        pad = LX - len(steps)
         dirs, ks = 0, 0
         for s in range(LX):
-            d, k = steps[s] if s < len(steps) else (3, 0)
+            d, k = (3, 0) if s < pad else steps[s - pad]
             dirs |= d << (2 * s)
             ks |= k << (4 * s)
         lane_tab.append((e[0], e[1], e[2], src[0], src[1], src[2], dirs, ks))
