@@ -216,7 +216,7 @@ def _chain_lines(LX, M, R="double", slot_pw=None):
     return L
 
 
-def _ket_lines(LX, LR, R="double", defer_top=False):
+def _ket_lines(LX, LR, R="double", defer_top=False, alt=False):
     """Ket OS VRR in registers: V{f}_{m} for |f| <= LR, m <= LR-|f| (type R).
 
     defer_top: the top level |f| = LR leaves out its cross-lane term
@@ -225,6 +225,10 @@ def _ket_lines(LX, LR, R="double", defer_top=False):
     o2s (e 0|f' 0)^(1) per lane (f' of level LR-1) over the primitive quartets
     and adds e_i times the sum of lane e-1_i once per contracted quartet
     (top_coupling).
+
+    alt (la+lb = 2): the first level's neighbour value in direction y or z
+    comes from the lane's own second chain, alt_m = (1_ja 0|0 0)^(m), where
+    amask says the predecessor e - 1_i is off the lane's chain.
 
     Returns (lines, name(f, m))."""
     L = []
@@ -253,6 +257,8 @@ def _ket_lines(LX, LR, R="double", defer_top=False):
                         # builds e from e - 1_i0, i0 = first non-zero of e; i = x
                         # is i0 wherever e_x > 0, and with |e| <= 1 every i is)
                         expr += " + cpl%s * c%d_%d" % (ax, LX - 1, m + 1)
+                    elif n == 1 and alt:
+                        expr += " + cpl%s * ((amask & %d) ? alt_%d : c%d_%d)" % (ax, 1 << i, m + 1, LX - 1, m + 1)
                     else:
                         expr += " + cpl%s * %s(%s, s4%s)" % (ax, shfl, V(f1, m + 1), ax)
                 L.append("const %s %s = %s;" % (R, V(t, m), expr))
@@ -345,7 +351,18 @@ def emit_class_source(cls, spherical, prec="d", waves_per_eu=None, batch=None):
             d, k = (3, 0) if s < pad else steps[s - pad]
             dirs |= d << (2 * s)
             ks |= k << (4 * s)
-        lane_tab.append((e[0], e[1], e[2], src[0], src[1], src[2], dirs, ks))
+        # first-level neighbour values off the lane's chain: for i with e_i > 0
+        # and i != i0, the predecessor e - 1_i; with la+lb = 2 that is at most
+        # one |p| = 1 component per lane, (1_ja 0|0 0) = one chain step
+        alt_dir, alt_mask, alt_p = 3, 0, None
+        i0 = first_nonzero(e)
+        for i in range(3 if LX == 2 else 0):
+            if e[i] > 0 and i != i0:
+                p = dec(e, i)
+                assert alt_p in (None, p) and sum(p) == 1
+                alt_p, alt_mask = p, alt_mask | (1 << i)
+                alt_dir = first_nonzero(p)
+        lane_tab.append((e[0], e[1], e[2], src[0], src[1], src[2], dirs, ks, alt_dir, alt_mask))
 
     o = Emitter()
     o.ind = 0
@@ -395,7 +412,7 @@ def emit_class_source(cls, spherical, prec="d", waves_per_eu=None, batch=None):
     # The body is shared by the per-class kernels below and by the persistent
     # multi-class kernel, which includes this file with INTC_BODIES_ONLY
     # defined (file-scope names have internal linkage for that reason).
-    o("static __constant__ int %s_lane[%d][8] = {" % (name, G))
+    o("static __constant__ int %s_lane[%d][10] = {" % (name, G))
     for r in lane_tab:
         o("    {%s}," % ", ".join(str(v) for v in r))
     o("};")
@@ -433,7 +450,8 @@ def emit_class_source(cls, spherical, prec="d", waves_per_eu=None, batch=None):
     o("const %s ex_d = (%s)lt[0], ey_d = (%s)lt[1], ez_d = (%s)lt[2];" % (R, R, R, R))
     o("const int lbase = lane - e;")
     o("const int s4x = (lbase + lt[3]) * 4, s4y = (lbase + lt[4]) * 4, s4z = (lbase + lt[5]) * 4;")
-    o("const int cdir = lt[6], ckk = lt[7];")
+    o("const int cdir = lt[6], ckk = lt[7], adir = lt[8], amask = lt[9];")
+    o("(void)adir; (void)amask;")
     o("(void)ex_d; (void)ey_d; (void)ez_d; (void)s4x; (void)s4y; (void)s4z; (void)cdir; (void)ckk;")
     o("const int nq = a.nq_dev ? *a.nq_dev : a.nq;")
     o("if (qend > nq) qend = nq;")
@@ -529,13 +547,31 @@ def emit_class_source(cls, spherical, prec="d", waves_per_eu=None, batch=None):
             for m in range(M + 1):
                 o("const double c0_%d = pref * F[%d];" % (m, m))
 
+    # second chain of the lanes whose first-level neighbour is off their own
+    # chain (la+lb = 2, a ket with a first level below its top)
+    use_alt = LX == 2 and LR >= 1 and not (DEFER_TOP and LR == 1)
+
+    def _alt_lines(R, slot_pw):
+        """alt_m = (1_ja 0|0 0)^(m) = PA_ja c0_m + WP_ja c0_{m+1}, m = 1..LR"""
+        L = []
+        if slot_pw:
+            L.append("const %s apa = (%s)u[%d + adir], awp = (%s)u[%d + adir];" % (R, R, slot_pw[0], R, slot_pw[1]))
+        else:
+            one, zero = ("1.0f", "0.0f") if R == "float" else ("1.0", "0.0")
+            L.append("const %s apa = adir == 0 ? PAx : (adir == 1 ? PAy : (adir == 2 ? PAz : %s));" % (R, one))
+            L.append("const %s awp = adir == 0 ? WPx : (adir == 1 ? WPy : (adir == 2 ? WPz : %s));" % (R, zero))
+        for m in range(1, LR + 1):
+            L.append("const %s alt_%d = apa * c0_%d + awp * c0_%d;" % (R, m, m, m + 1))
+        return L
+
     def emit_recurrences(slot_pw=None):
         """bra chain + ket VRR + accumulation on the names of the setup"""
         if LX > 0 and LR > 0:
             o("const %s cplx = ex_d * o2s, cply = ey_d * o2s, cplz = ez_d * o2s;" % R)
         if prec == "f":
             body = _chain_lines(LX, M, R, slot_pw)
-            lines, V = _ket_lines(LX, LR, R, defer)
+            body += _alt_lines(R, slot_pw) if use_alt else []
+            lines, V = _ket_lines(LX, LR, R, defer, use_alt)
             body += lines
             body = [ln.replace("o2z", "o2zf").replace("o2e", "o2ef") for ln in body]
             for ln in body:
@@ -550,7 +586,9 @@ def emit_class_source(cls, spherical, prec="d", waves_per_eu=None, batch=None):
         else:
             for ln in _chain_lines(LX, M, "double", slot_pw):
                 o(ln)
-            lines, V = _ket_lines(LX, LR, "double", defer)
+            for ln in (_alt_lines("double", slot_pw) if use_alt else []):
+                o(ln)
+            lines, V = _ket_lines(LX, LR, "double", defer, use_alt)
             for ln in lines:
                 o(ln)
             for f1 in defer_src:
