@@ -31,11 +31,11 @@ FLAGS = ["-O3", "-fPIC", "-std=c++17", "--offload-arch=" + ARCH,
 # with no spills, for 3 % more instructions in the recurrence loop
 CLASS_FLAGS = ["-mllvm", "-disable-machine-licm"]
 
-# occupancy targets (waves per SIMD) for class kernels that reach them with at
-# most a few bytes of spill and run faster there (config-3 bins on MI355X:
-# (ps|dp) 3.32 -> 3.06 ms, (ds|dp) 1.75 -> 1.64 ms); the compiler otherwise
-# settles at one wave fewer
-OCCUPANCY = {"intc_eri_d1021_s": 3, "intc_eri_d2021_s": 3}
+# occupancy targets (waves per SIMD) for class kernels that reach them without
+# spilling (no scratch memory; tests/test_build_artifacts.py checks every class
+# kernel) and run faster there (config-3 bins on MI355X: (ps|dp) 3.32 -> 3.06
+# ms, (ds|dp) 1.75 -> 1.64 ms); the compiler otherwise settles at one wave fewer
+OCCUPANCY = {"intc_eri_d1021_s": 3, "intc_eri_d2021_s": 3, "intc_eri_d2121_s": 3}
 # batched-setup choice where it differs from the default rule (groups of >= 4
 # lanes, ket of 2-3 levels), measured per kernel on the config-3 bins: the
 # FP32-recurrence kernels of these classes gain from it

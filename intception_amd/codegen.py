@@ -712,6 +712,12 @@ def emit_class_source(cls, spherical, prec="d", waves_per_eu=None, batch=None):
     # ket HRR + ket sph by group 0 lanes with |e| >= LA
     o("if (grp == 0 && e >= %d) {" % eoff)
     o.ind = 3
+    # the lane's LDS row: an offset opaque to the compiler, so that the row's
+    # addresses are formed here from one base (non-negative immediate offsets)
+    # rather than hoisted out of the quartet loop as NCD live registers
+    o("int trow_off = (e - %d) * NCD;" % eoff)
+    o('asm volatile("" : "+v"(trow_off));')
+    o("double *trow = tlds + trow_off;")
     if LD > 0:
         o("const double CDx = a.pair_AB[3 * pk], CDy = a.pair_AB[3 * pk + 1], CDz = a.pair_AB[3 * pk + 2];")
     hl, H = _hrr_lines(lambda f: "acc%d" % cidx(f), LC, LD, "CD", "hk", "ket")
@@ -729,7 +735,7 @@ def emit_class_source(cls, spherical, prec="d", waves_per_eu=None, batch=None):
         for sc in range(nsph(LC)):
             for sd in range(nsph(LD)):
                 terms = _sparse_terms(["k1_%d_%d" % (ci, sd) for ci in range(len(cc))], SPH_BUILTIN[LC][:, sc])
-                o("tlds[(e - %d) * NCD + %d] = %s;" % (eoff, sc * nD + sd, terms))
+                o("trow[%d] = %s;" % (sc * nD + sd, terms))
         o.ind -= 1
         o("} else {")
         o.ind += 1
@@ -743,13 +749,13 @@ def emit_class_source(cls, spherical, prec="d", waves_per_eu=None, batch=None):
         for sc in range(nsph(LC)):
             for sd in range(nsph(LD)):
                 terms = " + ".join("SC[%d] * k1_%d_%d" % (ci + sc * ncc(LC), ci, sd) for ci in range(len(cc)))
-                o("tlds[(e - %d) * NCD + %d] = %s;" % (eoff, sc * nD + sd, terms))
+                o("trow[%d] = %s;" % (sc * nD + sd, terms))
         o.ind -= 1
         o("}")
     else:
         for ci, c in enumerate(cc):
             for di, d in enumerate(dc):
-                o("tlds[(e - %d) * NCD + %d] = %s;" % (eoff, ci * nD + di, H(c, d)))
+                o("trow[%d] = %s;" % (ci * nD + di, H(c, d)))
     o.ind = 2
     o("}")
     o('__builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");')
