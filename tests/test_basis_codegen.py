@@ -124,7 +124,8 @@ def test_float_slot_layout():
     src, meta = codegen.emit_class_source((1, 1, 2, 1), True, "f")
     assert meta["NSF"] % 2 == 0 and meta["NS"] * 2 == meta["NSF"]
     step = src[src.index("const float *u = gslot + st * NSF;"):src.index("acc4 = fma(pref")]
-    assert "(float)u[" not in step.replace("(float)u[8 + d", "").replace("(float)u[12 + d", "")
+    # the only FP32 slot reads with an index are the PA/WP rows (offsets 8 and 12) by chain direction
+    assert "(float)u[" not in re.sub(r"\(float\)u\[(8|12) \+ (d\d|adir)\]", "", step)
     assert "const double pref = *(const double *)(u + 0);" in step
     src, meta = codegen.emit_class_source((1, 1, 2, 1), True, "d")
     assert meta["NSF"] == 0 and "const double *u = gslot + st * NS;" in src
