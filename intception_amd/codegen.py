@@ -129,6 +129,9 @@ SLOT_PW = True
 # the top ket level's cross-lane term is added once per contracted quartet
 # (_ket_lines defer_top) instead of once per primitive quartet
 DEFER_TOP = True
+# FP64 kernels: the top ket level goes into its contraction sums by fused
+# multiply-adds instead of being formed and then added
+FUSE_TOP = True
 
 
 def _lds_conflict(addrs):
@@ -216,7 +219,7 @@ def _chain_lines(LX, M, R="double", slot_pw=None):
     return L
 
 
-def _ket_lines(LX, LR, R="double", defer_top=False, alt=False):
+def _ket_lines(LX, LR, R="double", defer_top=False, alt=False, fuse_top=None):
     """Ket OS VRR in registers: V{f}_{m} for |f| <= LR, m <= LR-|f| (type R).
 
     defer_top: the top level |f| = LR leaves out its cross-lane term
@@ -229,6 +232,11 @@ def _ket_lines(LX, LR, R="double", defer_top=False, alt=False):
     alt (la+lb = 2): the first level's neighbour value in direction y or z
     comes from the lane's own second chain, alt_m = (1_ja 0|0 0)^(m), where
     amask says the predecessor e - 1_i is off the lane's chain.
+
+    fuse_top (FP64 only): name(f) of the contraction sum of each top-level
+    value; the top level is then not formed, its terms go into the sum by one
+    fused multiply-add each (one addition fewer per value and primitive
+    quartet).
 
     Returns (lines, name(f, m))."""
     L = []
@@ -246,6 +254,15 @@ def _ket_lines(LX, LR, R="double", defer_top=False, alt=False):
             k1 = f1[i]
             ax = "xyz"[i]
             for m in range(LR - n + 1):
+                if fuse_top is not None and n == LR:
+                    a = fuse_top(t)
+                    L.append("%s = fma(QC%s, %s, %s);" % (a, ax, V(f1, m), a))
+                    L.append("%s = fma(WQ%s, %s, %s);" % (a, ax, V(f1, m + 1), a))
+                    if k1 > 0:
+                        f2 = dec(f1, i)
+                        L.append("%s = fma(%d.0 * o2e, %s - roe * %s, %s);" % (a, k1, V(f2, m), V(f2, m + 1), a))
+                    assert LX == 0 or defer_top, "fused top level without its deferred coupling"
+                    continue
                 expr = "QC%s * %s + WQ%s * %s" % (ax, V(f1, m), ax, V(f1, m + 1))
                 if k1 > 0:
                     f2 = dec(f1, i)
@@ -588,13 +605,16 @@ def emit_class_source(cls, spherical, prec="d", waves_per_eu=None, batch=None):
                 o(ln)
             for ln in (_alt_lines("double", slot_pw) if use_alt else []):
                 o(ln)
-            lines, V = _ket_lines(LX, LR, "double", defer, use_alt)
+            fuse = FUSE_TOP and LR > 0 and (LX == 0 or defer)
+            lines, V = _ket_lines(LX, LR, "double", defer, use_alt,
+                                  (lambda f: "acc%d" % cidx(f)) if fuse else None)
             for ln in lines:
                 o(ln)
             for f1 in defer_src:
                 o("bq%d = fma(o2s, %s, bq%d);" % (cidx(f1), V(f1, 1), cidx(f1)))
             for f in acc_f:
-                o("acc%d += %s;" % (cidx(f), V(f, 0)))
+                if not (fuse and sum(f) == LR):
+                    o("acc%d += %s;" % (cidx(f), V(f, 0)))
 
     if not batched:
         o("int ib = grp / Pk, ik = grp - ib * Pk;")

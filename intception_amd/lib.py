@@ -84,6 +84,11 @@ def load():
     if not os.path.exists(LIB_PATH):
         raise IntcError("libintc.so is not built (run intception_amd.build.build()); "
                         "the ERI path has no CPU fallback")
+    # torch first: libintc then binds to the HIP runtime torch has loaded, the
+    # one that owns the device memory and streams it hands over.  Loaded the
+    # other way round, the system HIP runtime takes the shared soname and
+    # torch's own device initialisation fails ("No HIP GPUs are available").
+    import torch  # noqa: F401
     L = ctypes.CDLL(LIB_PATH)
     for name, res, args in SIGNATURES:
         f = getattr(L, name)

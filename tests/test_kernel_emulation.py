@@ -176,3 +176,42 @@ def test_class_bodies_on_the_host(tmp_path):
         rel = np.abs(got - ref).max() / np.abs(ref).max()
         print(case, "max err / block max %.2e" % rel)
         assert rel < (1e-11 if case[1] == "d" else 1e-3), case
+
+
+@pytest.mark.skipif(GXX is None, reason="needs g++")
+def test_queue_entries_outside_the_bin_store_nothing(tmp_path):
+    """Routed queue entries outside the bin (negative or >= nq) are passed over:
+    their teams store nothing and queue nothing for the FP64 redo, and every
+    other quartet of the queue is evaluated as usual (FP64 and FP32 bodies, one
+    and two groups per team, the class kernels' pipelined loop and the
+    persistent kernel's loop)."""
+    tmp = str(tmp_path)
+    oracle.set_boys_mode(oracle.BOYS_EXACT)
+    nq = 16
+    for cls, prec, T, bodies in (((1, 1, 2, 1), "d", 1, 0), ((1, 0, 1, 0), "f", 2, 0), ((1, 1, 2, 1), "d", 1, 1)):
+        exe, _ = _build(tmp, cls, prec, bodies)
+        b, parts = _inputs(cls, nq, 3, T, 0)
+        blk = int(parts[0][3]) // nq
+        queue = parts[-1].copy()
+        dropped = queue[[1, 6, 7, 12]].copy()
+        queue[[1, 6, 7, 12]] = [-1, nq, -5, nq + 3]
+        parts = parts[:-1] + [queue]
+        inp, outp = os.path.join(tmp, "in_bad"), os.path.join(tmp, "out_bad")
+        with open(inp, "wb") as fh:
+            for arr in parts:
+                arr.tofile(fh)
+        args = [exe, inp, outp, str(T), "64", "1"] + (["1e-300"] if prec == "f" else [])
+        r = subprocess.run(args, capture_output=True, text=True,
+                           env=dict(os.environ, ASAN_OPTIONS="detect_leaks=0:abort_on_error=1"))
+        assert r.returncode == 0, r.stderr[-3000:]
+        raw = np.fromfile(outp, dtype=np.uint8)
+        got = raw[:8 * nq * blk].view(np.float64).reshape(nq, blk)
+        ref = oracle.eri_batch(b, True, basis.SPH).reshape(nq, blk)
+        keep = np.setdiff1d(np.arange(nq), dropped)
+        assert np.all(got[dropped] == 12345.0), (cls, prec, bodies)
+        rel = np.abs(got[keep] - ref[keep]).max() / np.abs(ref[keep]).max()
+        assert rel < (1e-11 if prec == "d" else 1e-3), (cls, prec, bodies, rel)
+        if prec == "f":
+            n = int(raw[8 * nq * blk:8 * nq * blk + 4].view(np.int32)[0])
+            q = raw[8 * nq * blk + 4:].view(np.int32)
+            assert n == nq - len(dropped) and sorted(q.tolist()) == sorted(keep.tolist()), (n, q)
