@@ -782,9 +782,11 @@ def run_config2(args):
     n_int = batch.out_size
     res = {
         "metric": "contracted ERIs/s on 1 MI355X (config 2, high angular momentum; FP64 and FP32 recurrences)",
-        "value": float(n_int / (ms32 * 1e-3)), "unit": "integrals/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": float(ms32), "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f32 recurrences + f64 Boys/prefactors/contraction/HRR/transforms",
+        # the FP64 path: the FP32 recurrences miss every error bar on these
+        # classes (their errors are reported below) and are no faster
+        "value": float(n_int / (ms64 * 1e-3)), "unit": "integrals/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": float(ms64), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": "config2: %d quartets, (dd|dd),(fd|fd),(ff|dd),(ff|ff), K 1..3, exponents "
                                "10^U[log 0.05, log 50), spherical" % batch.n_quartet,

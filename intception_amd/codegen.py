@@ -118,11 +118,10 @@ class Emitter:
 
 # groups of at least this many lanes whose ket reaches lc + ld in BATCH_LK use
 # the batched setup loop (measured on MI355X, config-3 bins: 1.06-1.37x for
-# ket 2-3; for a one-level ket the recurrences are too short to pay for the
-# LDS round trip, and the ket-4 kernels sit at the register limit); a
-# per-kernel choice (emit_class_source's batch) overrides the rule
+# ket 2-3, 2.0x for (ps|ps), 1.05x for (pp|dd) and (ps|dd)); a per-kernel
+# choice (emit_class_source's batch) overrides the rule
 BATCH_MIN_G = 4
-BATCH_LK = (2, 3)
+BATCH_LK = (1, 2, 3, 4)
 # batched kernels: each lane reads its bra-chain step's PA / WP entry from the
 # setup slot (instead of selecting among the x, y, z entries)
 SLOT_PW = True
