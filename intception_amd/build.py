@@ -36,11 +36,9 @@ CLASS_FLAGS = ["-mllvm", "-disable-machine-licm"]
 # kernel) and run faster there (config-3 bins on MI355X: (ps|dp) 3.32 -> 3.06
 # ms, (ds|dp) 1.75 -> 1.64 ms); the compiler otherwise settles at one wave fewer
 OCCUPANCY = {"intc_eri_d1021_s": 3, "intc_eri_d2021_s": 3, "intc_eri_d2121_s": 3}
-# batched-setup choice where it differs from the default rule (groups of >= 4
-# lanes, ket of 2-3 levels), measured per kernel on the config-3 bins: the
-# FP32-recurrence kernels of these classes gain from it
-# ((ps|ps) 2.81 -> 2.03 ms, (pp|dd) 2.03 -> 1.82 ms, (ds|dd), (dp|dd))
-BATCH = {"intc_eri_f1010_s": True, "intc_eri_f1122_s": True, "intc_eri_f2022_s": True, "intc_eri_f2122_s": True}
+# batched-setup choice per kernel where it differs from the default rule
+# (codegen.BATCH_MIN_G, BATCH_LK: every group of >= 4 lanes); none at present
+BATCH = {}
 
 # classes compiled into the library: every canonical class with shells up to
 # l = 2 (Cartesian and spherical), plus the three-centre set (ab|P s) with

@@ -107,6 +107,9 @@ __device__ __forceinline__ void intc_boys(double T, double *F, const Args &a)
     // index below in range whatever the input
     T = fmax(T, 0.0);
     if (BM == 0) {
+        // exp(-T) once, ahead of the branch: a wave whose lanes take both
+        // branches would otherwise evaluate it twice
+        const double ex = M > 0 ? exp(-T) : 0.0;
         if (T < (double)INTC_BOYS_NINT) {
             int k = (int)T;
             double t = 2.0 * (T - (double)k) - 1.0;
@@ -116,7 +119,7 @@ __device__ __forceinline__ void intc_boys(double T, double *F, const Args &a)
             for (int j = INTC_BOYS_DEG - 1; j >= 0; --j) v = fma(v, t, c[j]);
             F[M] = v;
             if (M > 0) {
-                double ex = exp(-T), t2 = 2.0 * T;
+                double t2 = 2.0 * T;
 #pragma unroll
                 for (int m = M - 1; m >= 0; --m) F[m] = fma(t2, F[m + 1], ex) * (1.0 / (2 * m + 1));
             }
@@ -124,7 +127,7 @@ __device__ __forceinline__ void intc_boys(double T, double *F, const Args &a)
             double it = 1.0 / T;
             F[0] = 0.5 * sqrt(3.141592653589793238 * it);
             if (M > 0) {
-                double ex = exp(-T), h = 0.5 * it;
+                double h = 0.5 * it;
 #pragma unroll
                 for (int m = 0; m < M; ++m) F[m + 1] = fma((double)(2 * m + 1), F[m], -ex) * h;
             }
