@@ -54,10 +54,17 @@ def _digest(paths, extra=""):
     return h.hexdigest()
 
 
+# the network's kernels: MFMA results in VGPRs (not AGPRs), so the router's
+# activations need no accumulator-to-VGPR copies between its layers
+MLP_FLAGS = ["-mllvm", "-amdgpu-mfma-vgpr-form"]
+
+
 def _flags(src):
-    """compile flags of one source: the generated class kernels add CLASS_FLAGS"""
+    """compile flags of one source: the generated class kernels add
+    CLASS_FLAGS, the network's source MLP_FLAGS"""
     gen = os.path.dirname(os.path.abspath(src)) == GEN and os.path.basename(src).startswith("intc_eri_")
-    return FLAGS + (CLASS_FLAGS if gen else [])
+    mlp = os.path.basename(src) == "intc_mlp.hip"
+    return FLAGS + (CLASS_FLAGS if gen else []) + (MLP_FLAGS if mlp else [])
 
 
 def _compile(src, deps, force=False):

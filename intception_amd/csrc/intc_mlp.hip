@@ -128,8 +128,8 @@ __global__ void __launch_bounds__(256) intc_mlp_features_k(long long n, const fl
 // next layer's B operand in registers:
 //   H1^T = relu(W1 X^T + b1)      v_mfma_f32_16x16x16_bf16, 4 row tiles
 //   H2^T = relu(W2 H1^T + b2)     v_mfma_f32_16x16x32_bf16, 4 row tiles x 2 K chunks
-//   y    = w3 . bf16(H2^T) + b3   per lane over its 16 hidden units, then
-//                                 across the 4 lane groups
+//   y    = w3 . bf16(H2^T) + b3   v_mfma_f32_16x16x32_bf16, 2 K chunks, with
+//                                 w3 in every row of A (each row of D is y)
 // Operands (lane l, r = l % 16, g = l / 16, slot s):
 //   A[i][k]: i = r; B[k][n]: n = r; D[i][n]: n = r, i = 4 g + s (s < 4).
 // The K index of a slot is the same function of (g, s) in A and B, so the
@@ -142,8 +142,6 @@ __global__ void __launch_bounds__(256) intc_mlp_features_k(long long n, const fl
 constexpr int SF = KF + 1;  // LDS row: 16 standardised features (zero padded), padded to 17 words
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-
-__device__ __forceinline__ float bf16_round(float x) { return (float)(__bf16)x; }
 
 // max(x, 0) on the bit pattern: one integer max, no NaN canonicalisation
 __device__ __forceinline__ float relu_bits(float x) { return __int_as_float(max(__float_as_int(x), 0)); }
@@ -161,9 +159,8 @@ __global__ void __launch_bounds__(256) intc_mlp_route_k(long long n, const float
     // A operands of layers 1 and 2 (rows 16 nt + r16), and the biases and
     // W3 entries of the hidden units this lane's accumulator slots hold
     bf16x4 a1w[4];
-    bf16x8 a2w[4][2];
+    bf16x8 a2w[4][2], a3w[2];
     f32x4 b1v[4], b2v[4];
-    float w3r[4][4];
     for (int nt = 0; nt < 4; ++nt) {
         const int row = 16 * nt + r16;
         for (int j = 0; j < 4; ++j) a1w[nt][j] = (short)wb[BW1 + row * KF + 4 * g4 + j];
@@ -174,9 +171,13 @@ __global__ void __launch_bounds__(256) intc_mlp_route_k(long long n, const float
             const int h = 16 * nt + 4 * g4 + j;
             b1v[nt][j] = master[OB1 + h];
             b2v[nt][j] = master[OB2 + h];
-            w3r[nt][j] = bf2f(wb[BW3 + h]);
         }
     }
+    // A of layer 3: w3 in every row, K chunk c slot s = hidden-2 unit
+    // 16 (2c + s/4) + 4 g4 + s%4 (the units layer 2's D holds there)
+    for (int c = 0; c < 2; ++c)
+        for (int sl = 0; sl < 8; ++sl)
+            a3w[c][sl] = __builtin_bit_cast(__bf16, wb[BW3 + 16 * (2 * c + sl / 4) + 4 * g4 + sl % 4]);
     const float b3 = master[OB3];
     const long long nsuper = (n + 63) / 64;
     const long long stride = (long long)gridDim.x * (blockDim.x >> 6);
@@ -221,18 +222,20 @@ __global__ void __launch_bounds__(256) intc_mlp_route_k(long long n, const float
                     const float h = d1[2 * c + sl / 4][sl % 4];
                     h1[c][sl] = (__bf16)relu_bits(h);
                 }
-            float part = 0.0f;
+            f32x4 d2[4];
             for (int nt = 0; nt < 4; ++nt) {
-                f32x4 d2 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2w[nt][0], h1[0], b2v[nt], 0, 0, 0);
-                d2 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2w[nt][1], h1[1], d2, 0, 0, 0);
-                for (int j = 0; j < 4; ++j) {
-                    const float h = d2[j];
-                    part += bf16_round(relu_bits(h)) * w3r[nt][j];
-                }
+                d2[nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2w[nt][0], h1[0], b2v[nt], 0, 0, 0);
+                d2[nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2w[nt][1], h1[1], d2[nt], 0, 0, 0);
             }
-            part += __shfl_xor(part, 16);
-            part += __shfl_xor(part, 32);
-            if (g4 == tt) mine = part;  // lane 16 tt + r16 keeps quartet 16 tt + r16
+            // B of layer 3: relu(H2) in BF16, in the K order of layer 2's B
+            bf16x8 h2[2];
+            for (int c = 0; c < 2; ++c)
+                for (int sl = 0; sl < 8; ++sl)
+                    h2[c][sl] = (__bf16)relu_bits(d2[2 * c + sl / 4][sl % 4]);
+            const f32x4 zero = {0.0f, 0.0f, 0.0f, 0.0f};
+            f32x4 d3 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a3w[0], h2[0], zero, 0, 0, 0);
+            d3 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a3w[1], h2[1], d3, 0, 0, 0);
+            if (g4 == tt) mine = d3[0];  // lane 16 tt + r16 keeps quartet 16 tt + r16
         }
         if (q < n) {
             // the network predicts log10(max|integral| / (Q_bra Q_ket)) <= 0
