@@ -524,11 +524,12 @@ def emit_class_source(cls, spherical, prec="d", waves_per_eu=None, batch=None):
         o("(void)o2z; (void)o2e;")
         if load_idx:
             o("ik += T; while (ik >= Pk) { ik -= Pk; ++ib; }")
-        o("const double os = 1.0 / (zeta + eta);")
+        # 1/sqrt(zeta + eta) gives both 1/(zeta + eta) and the prefactor's root
+        o("const double ros = intc_rsqrt(zeta + eta), os = ros * ros;")
         o("const double rho = zeta * eta * os;")
         o("const double PQx = Px - Qx, PQy = Py - Qy, PQz = Pz - Qz;")
         o("const double Tb = rho * (PQx * PQx + PQy * PQy + PQz * PQz);")
-        o("const double pref = kab * kcd * sqrt(os);")
+        o("const double pref = kab * kcd * ros;")
         o("double F[M + 1];")
         o("intc_boys<M, BM>(Tb, F, a);")
         if prec == "f":

@@ -619,11 +619,11 @@ __global__ void __launch_bounds__(WG_BLOCK) intc_eri_wg_k(const IntcEriArgs a, c
                 const double eta = a.pp_zeta[okk + ik], o2e = a.pp_o2z[okk + ik];
                 const double Qx = a.pp_px[okk + ik], Qy = a.pp_py[okk + ik], Qz = a.pp_pz[okk + ik];
                 const double kcd = a.pp_kappa[okk + ik];
-                const double os = 1.0 / (zeta + eta);
+                const double ros = intc_rsqrt(zeta + eta), os = ros * ros;
                 const double rho = zeta * eta * os;
                 const double PQx = Px - Qx, PQy = Py - Qy, PQz = Pz - Qz;
                 const double Tb = rho * (PQx * PQx + PQy * PQy + PQz * PQz);
-                const double pref = kab * kcd * sqrt(os);
+                const double pref = kab * kcd * ros;
                 const bool f32 = sizeof(VT) == 4;
                 const double sc = (f32 && Tb > 0.5) ? 2.0 * Tb : 1.0, isc = 1.0 / sc;
                 const double eo = eta * os, zo = zeta * os;

@@ -49,6 +49,9 @@ inline int emu_bpermute(int addr, int v)
 #define __builtin_amdgcn_ds_bpermute(a, v) emu_bpermute((a), (v))
 #define __builtin_amdgcn_wave_barrier() g_team.bar->arrive_and_wait()
 #define __builtin_amdgcn_fence(a, b) ((void)0)
+// the hardware reciprocal square root is an estimate: stand in with a
+// single-precision one, so the Newton steps that follow it are exercised
+#define __builtin_amdgcn_rsq(x) ((double)(1.0f / sqrtf((float)(x))))
 // device atomics: the lanes of the team are host threads
 inline int atomicAdd(int *p, int v) { return __atomic_fetch_add(p, v, __ATOMIC_SEQ_CST); }
 inline int __double2loint(double d) { uint64_t u; memcpy(&u, &d, 8); return (int)(uint32_t)u; }
