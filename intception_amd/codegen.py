@@ -515,12 +515,14 @@ def emit_class_source(cls, spherical, prec="d", waves_per_eu=None, batch=None):
         o("double bq%d = 0.0;" % cidx(f1))
     def emit_setup(load_idx):
         """pair loads + Boys + coefficients for primitive quartet (ib, ik)"""
-        o("const double zeta = a.pp_zeta[ob + ib], o2z = a.pp_o2z[ob + ib];")
-        o("const double Px = a.pp_px[ob + ib], Py = a.pp_py[ob + ib], Pz = a.pp_pz[ob + ib];")
-        o("const double kab = a.pp_kappa[ob + ib];")
-        o("const double eta = a.pp_zeta[okk + ik], o2e = a.pp_o2z[okk + ik];")
-        o("const double Qx = a.pp_px[okk + ik], Qy = a.pp_py[okk + ik], Qz = a.pp_pz[okk + ik];")
-        o("const double kcd = a.pp_kappa[okk + ik];")
+        # one 32-bit byte offset per primitive pair, shared by its six arrays
+        o("const unsigned bo = (unsigned)(ob + ib) * 8u, ko = (unsigned)(okk + ik) * 8u;")
+        o("const double zeta = intc_ld(a.pp_zeta, bo), o2z = intc_ld(a.pp_o2z, bo);")
+        o("const double Px = intc_ld(a.pp_px, bo), Py = intc_ld(a.pp_py, bo), Pz = intc_ld(a.pp_pz, bo);")
+        o("const double kab = intc_ld(a.pp_kappa, bo);")
+        o("const double eta = intc_ld(a.pp_zeta, ko), o2e = intc_ld(a.pp_o2z, ko);")
+        o("const double Qx = intc_ld(a.pp_px, ko), Qy = intc_ld(a.pp_py, ko), Qz = intc_ld(a.pp_pz, ko);")
+        o("const double kcd = intc_ld(a.pp_kappa, ko);")
         o("(void)o2z; (void)o2e;")
         if load_idx:
             o("ik += T; while (ik >= Pk) { ik -= Pk; ++ib; }")
