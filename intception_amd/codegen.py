@@ -491,15 +491,15 @@ def emit_class_source(cls, spherical, prec="d", waves_per_eu=None, batch=None):
     o("for (int qq = qq0 + tw; qq < qend; qq += qstep) {")
     o("    const int q = q1, pb = pb1, pk = pk1;")
     o("    q1 = q2;")
-    o("    if (q1 >= 0 && q1 < a.nq) { pb1 = a.q_bra[q1]; pk1 = a.q_ket[q1]; }")
+    o("    if (q1 >= 0 && q1 < a.nq) { const unsigned o4 = (unsigned)q1 * 4u; pb1 = intc_ld(a.q_bra, o4); pk1 = intc_ld(a.q_ket, o4); }")
     o("    q2 = qq + 2 * qstep < qend ? (a.q_idx ? a.q_idx[qq + 2 * qstep] : qq + 2 * qstep) : -1;")
     o("    if (q < 0 || q >= a.nq) continue;  // queue entries index the bin")
-    o("    const long long qoff = a.q_off[q];")
-    o("    const unsigned long long qs = a.q_str[q];")
+    o("    const long long qoff = intc_ld(a.q_off, (unsigned)q * 8u);")
+    o("    const unsigned long long qs = intc_ld(a.q_str, (unsigned)q * 8u);")
     o("#endif")
     o.ind = 2
-    o("const int ob = a.pair_off[pb], Pb = a.pair_np[pb];")
-    o("const int okk = a.pair_off[pk], Pk = a.pair_np[pk];")
+    o("const int ob = intc_ld(a.pair_off, (unsigned)pb * 4u), Pb = intc_ld(a.pair_np, (unsigned)pb * 4u);")
+    o("const int okk = intc_ld(a.pair_off, (unsigned)pk * 4u), Pk = intc_ld(a.pair_np, (unsigned)pk * 4u);")
     o("const double Ax = a.pair_A[3 * pb], Ay = a.pair_A[3 * pb + 1], Az = a.pair_A[3 * pb + 2];")
     o("const double Cx = a.pair_A[3 * pk], Cy = a.pair_A[3 * pk + 1], Cz = a.pair_A[3 * pk + 2];")
     o("(void)Ax; (void)Ay; (void)Az; (void)Cx; (void)Cy; (void)Cz;")

@@ -93,12 +93,13 @@ __device__ __forceinline__ float intc_shflf(float v, int src4)
     return __int_as_float(__builtin_amdgcn_ds_bpermute(src4, __float_as_int(v)));
 }
 
-// p[i] read through the 32-bit byte offset 8 i (i < 2^29): loads from several
+// p[i] read through the 32-bit byte offset i sizeof(T) (below 2^32): loads from several
 // arrays at one index then share one offset register (scalar base + vector
 // offset addressing) instead of forming a 64-bit address per array
-__device__ __forceinline__ double intc_ld(const double *p, unsigned byte_off)
+template <class T>
+__device__ __forceinline__ T intc_ld(const T *p, unsigned byte_off)
 {
-    return *(const double *)((const char *)p + byte_off);
+    return *(const T *)((const char *)p + byte_off);
 }
 
 // 1/sqrt(x) for normal x > 0: the hardware estimate (v_rsq_f64, relative
