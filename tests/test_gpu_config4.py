@@ -65,7 +65,7 @@ def test_full_three_center_learned_routing(mol4):
     # the benchmark's router (training and calibration sets as there)
     router, stats = screening.build_router(n_train=2_000_000, train_seeds=(202621, 202622),
                                            val_seed=(202629, 202628, 202627), n_val=1_000_000, n_atoms=100,
-                                           epochs=15, tau_skip=2.5e-11, three_center=True)
+                                           epochs=25, under_weight=100.0, tau_skip=2.5e-11, three_center=True)
     print("router built in %.1f s: val skip %.3f, val fp32 %.3f" % (time.perf_counter() - t0,
                                                                     stats["val_skip_fraction"],
                                                                     stats["val_fp32_fraction"]), flush=True)
