@@ -59,12 +59,20 @@ def _digest(paths, extra=""):
 MLP_FLAGS = ["-mllvm", "-amdgpu-mfma-vgpr-form"]
 
 
+# class kernels that keep machine-level loop-invariant motion: with it they
+# stay at the same waves per SIMD and without scratch (the hoisted constants
+# of the Boys setup then cost no instructions per primitive batch)
+LICM_CLASSES = {"intc_eri_d1011_s", "intc_eri_d1020_s", "intc_eri_d1022_s", "intc_eri_d1111_s",
+                "intc_eri_d1120_s", "intc_eri_d1122_s", "intc_eri_d0011_s", "intc_eri_d2022_s"}
+
+
 def _flags(src):
     """compile flags of one source: the generated class kernels add
-    CLASS_FLAGS, the network's source MLP_FLAGS"""
-    gen = os.path.dirname(os.path.abspath(src)) == GEN and os.path.basename(src).startswith("intc_eri_")
-    mlp = os.path.basename(src) == "intc_mlp.hip"
-    return FLAGS + (CLASS_FLAGS if gen else []) + (MLP_FLAGS if mlp else [])
+    CLASS_FLAGS (except LICM_CLASSES), the network's source MLP_FLAGS"""
+    base = os.path.splitext(os.path.basename(src))[0]
+    gen = os.path.dirname(os.path.abspath(src)) == GEN and base.startswith("intc_eri_")
+    mlp = base == "intc_mlp"
+    return FLAGS + (CLASS_FLAGS if gen and base not in LICM_CLASSES else []) + (MLP_FLAGS if mlp else [])
 
 
 def _compile(src, deps, force=False):
