@@ -138,7 +138,12 @@ __device__ __forceinline__ void intc_boys(double T, double *F, const Args &a)
             const double *c = a.boys_cheb + ((size_t)(M * INTC_BOYS_NINT + k)) * (INTC_BOYS_DEG + 1);
             double v = c[INTC_BOYS_DEG];
 #pragma unroll
-            for (int j = INTC_BOYS_DEG - 1; j >= 0; --j) v = fma(v, t, c[j]);
+            for (int j = INTC_BOYS_DEG - 1; j >= 0; --j) {
+                v = fma(v, t, c[j]);
+                // the upper half of the coefficients is consumed before the
+                // lower half is loaded: fewer of them in registers at once
+                if (j == INTC_BOYS_DEG / 2) __builtin_amdgcn_sched_barrier(0);
+            }
             F[M] = v;
             if (M > 0) {
                 double t2 = 2.0 * T;

@@ -49,6 +49,7 @@ inline int emu_bpermute(int addr, int v)
 #define __builtin_amdgcn_ds_bpermute(a, v) emu_bpermute((a), (v))
 #define __builtin_amdgcn_wave_barrier() g_team.bar->arrive_and_wait()
 #define __builtin_amdgcn_fence(a, b) ((void)0)
+#define __builtin_amdgcn_sched_barrier(m) ((void)0)
 // the hardware reciprocal square root is an estimate: stand in with a
 // single-precision one, so the Newton steps that follow it are exercised
 #define __builtin_amdgcn_rsq(x) ((double)(1.0f / sqrtf((float)(x))))
