@@ -105,7 +105,10 @@ def tune(plan, out, reps=3, boys_mode=0, grid_modes=("auto", 4, 8), save=True, l
                 for mode in grid_modes:
                     grid = grid_for(r["nq"], r["G"], T, mode)
                     plan.set_tuning(r["bin"], T, grid, prec)
-                    plan.run_bin(r["bin"], out, prec)  # warm
+                    # two untimed launches: after a synchronisation a bin's
+                    # launches start up to 15 % slower and settle over a few
+                    plan.run_bin(r["bin"], out, prec)
+                    plan.run_bin(r["bin"], out, prec)
                     ev0.record(stream)
                     for _ in range(reps):
                         plan.run_bin(r["bin"], out, prec)
