@@ -88,7 +88,7 @@ def G_ok(G, T):
     return 1 <= T and G * T <= 64
 
 
-def tune(plan, out, reps=3, boys_mode=0, grid_modes=("auto", 4, 8), save=True, log=None, save_path=None,
+def tune(plan, out, reps=3, boys_mode=0, grid_modes=("auto", 3, 4, 8, 16), save=True, log=None, save_path=None,
          precs=(0,)):
     """Time every candidate per bin (full bins) and set the fastest; returns the table."""
     import torch
