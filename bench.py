@@ -228,10 +228,10 @@ def run_molecule(args, three_center=False):
     train_seeds, val_seed, tau_skip = (((202621, 202622), (202629, 202628, 202627), 2.5e-11) if three_center else
                                        ((202611, 202612), (202619, 202618), 5e-11))
     # training length and under-prediction weight: sweeps on the held-out
-    # molecules gave the largest held-out skip shares at 20 epochs, weight 50,
-    # 3 M quartets for config 3 (0.433, against 0.421 at 15 epochs, weight 10)
+    # molecules gave the largest held-out skip shares at 25 epochs, weight 150,
+    # 3 M quartets for config 3 (0.442, against 0.421 at 15 epochs, weight 10)
     # and 25 epochs, weight 100, 2 M quartets for config 4 (0.139 against 0.102)
-    epochs, under_w, n_train = (25, 100.0, 2_000_000) if three_center else (20, 50.0, 3_000_000)
+    epochs, under_w, n_train = (25, 100.0, 2_000_000) if three_center else (25, 150.0, 3_000_000)
     router, rstats = screening.build_router(n_train=args.train or n_train, n_atoms=n_atoms, epochs=epochs,
                                             under_weight=under_w, n_val=1_000_000, log=log,
                                             three_center=three_center, train_seeds=train_seeds,

@@ -59,8 +59,8 @@ def test_full_molecule_learned_routing(mol3):
     # under-predicted a few rare one-centre (ds|ds)-type quartets by eight
     # decades and skipped blocks of up to 9e-9)
     router, stats = screening.build_router(n_train=3_000_000, train_seeds=(202611, 202612),
-                                           val_seed=(202619, 202618), n_val=1_000_000, n_atoms=200, epochs=20,
-                                           under_weight=50.0, tau_skip=5e-11)
+                                           val_seed=(202619, 202618), n_val=1_000_000, n_atoms=200, epochs=25,
+                                           under_weight=150.0, tau_skip=5e-11)
     print("router built in %.1f s: val skip %.3f, val fp32 %.3f" % (time.perf_counter() - t0,
                                                                     stats["val_skip_fraction"],
                                                                     stats["val_fp32_fraction"]), flush=True)
